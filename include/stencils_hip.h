@@ -1,0 +1,199 @@
+/*
+ * stencils_hip.h — C ABI of libstencils_hip.so, the MI355X (gfx950) drop-in
+ * for the `mapstencil` / `mapstencil!` hot path of Stencils.jl v0.3.6.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns an
+ * int status (STH_OK = 0) and never lets a C++ exception cross the boundary.
+ * The message of the last failure on the calling thread is sth_last_error().
+ *
+ * Array layout follows Julia: column-major, dim 1 contiguous.  An sth_array
+ * describes the LOGICAL array of a StencilArray: `size` is the logical size
+ * (src/array.jl:470-473), `data` the parent's first element, `halo` the
+ * number of padding cells on each side of every dim (R for Halo padding,
+ * 0 for Conditional), `stride` the parent's strides in elements.  Logical
+ * index i (0-based) of dim d lives at parent index i + halo.
+ *
+ * Status codes mirror the reference's failure modes:
+ *   STH_EINVAL  — ArgumentError in the reference (size mismatch
+ *                 src/gatherstencil.jl:121, radius >= axis src/array.jl:452,
+ *                 kernel length src/stencils/kernel.jl:115); the Julia shim
+ *                 rethrows ArgumentError with sth_last_error().
+ *   STH_ENOTSUP — a combination the GPU path does not accelerate; the shim
+ *                 falls back to the reference method (`invoke`).
+ *   STH_ENOMEM, STH_EDEVICE — allocation / HIP runtime failures.
+ */
+#ifndef STENCILS_HIP_H
+#define STENCILS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define STH_ABI_VERSION 1
+
+enum {
+  STH_OK = 0,
+  STH_EDEVICE = 5,
+  STH_ENOMEM = 12,
+  STH_EINVAL = 22,
+  STH_ENOTSUP = 95
+};
+
+/* Element types (Julia Bool, UInt8 ... Float64; BF16 is a storage type). */
+typedef enum {
+  STH_BOOL = 0,
+  STH_U8 = 1,
+  STH_I8 = 2,
+  STH_U16 = 3,
+  STH_I16 = 4,
+  STH_U32 = 5,
+  STH_I32 = 6,
+  STH_U64 = 7,
+  STH_I64 = 8,
+  STH_F16 = 9,
+  STH_BF16 = 10,
+  STH_F32 = 11,
+  STH_F64 = 12
+} sth_dtype;
+
+/* Boundary conditions, src/boundary.jl:17-52. */
+typedef enum {
+  STH_REMOVE = 0,  /* out-of-range neighbours read `padval`            */
+  STH_WRAP = 1,    /* i < 1 -> i + s ; i > s -> i - s                    */
+  STH_REFLECT = 2, /* i < 1 -> 2 - i ; i > s -> 2s - i (edge not repeated) */
+  STH_USE = 3      /* Halo padding only: read the halo as it is          */
+} sth_boundary;
+
+/* Padding modes, src/padding.jl:28-39. */
+typedef enum {
+  STH_CONDITIONAL = 0,
+  STH_HALO_OUT = 1,
+  STH_HALO_IN = 2
+} sth_padding;
+
+/* Stencil shape kinds (offset generators under src/stencils/). */
+typedef enum {
+  STH_SHAPE_POSITIONAL = 0,
+  STH_SHAPE_WINDOW = 1,
+  STH_SHAPE_MOORE = 2,
+  STH_SHAPE_VONNEUMANN = 3,
+  STH_SHAPE_CIRCLE = 4,
+  STH_SHAPE_CROSS = 5,
+  STH_SHAPE_ANGLEDCROSS = 6,
+  STH_SHAPE_FORWARDSLASH = 7,
+  STH_SHAPE_BACKSLASH = 8,
+  STH_SHAPE_VERTICAL = 9,
+  STH_SHAPE_HORIZONTAL = 10,
+  STH_SHAPE_DIAMOND = 11,
+  STH_SHAPE_ANNULUS = 12,
+  STH_SHAPE_CARDINAL = 13,
+  STH_SHAPE_ORDINAL = 14,
+  STH_SHAPE_RECTANGLE = 15,
+  STH_SHAPE_NAMED = 16
+} sth_shape;
+
+/* The fixed reduction set accelerated on the GPU. */
+typedef enum {
+  STH_OP_SUM = 0,           /* StaticArrays sum: left fold in offset order   */
+  STH_OP_MEAN = 1,          /* Statistics mean: fold in float(T), / L        */
+  STH_OP_MIN = 2,           /* minimum: left fold with Julia `min`           */
+  STH_OP_MAX = 3,           /* maximum: left fold with Julia `max`           */
+  STH_OP_KERNELPRODUCT = 4, /* kernelproduct, src/stencils/kernel.jl:37-43   */
+  STH_OP_LIFE = 5,          /* Life-like rule on the count of live neighbours */
+  STH_OP_LAPLACIAN = 6      /* c + alpha * (sum(neighbours) - L * c)          */
+} sth_opkind;
+
+/* sth_op.flags */
+#define STH_OPF_ACC_F32 1u /* BF16/F16 storage: widen to f32, round once at store */
+
+typedef struct sth_array {
+  void* data;         /* device pointer to the parent's first element */
+  int32_t dtype;      /* sth_dtype */
+  int32_t ndim;       /* 1..3 */
+  int64_t size[3];    /* logical size; unused dims are 1 */
+  int64_t stride[3];  /* parent strides in elements */
+  int32_t halo;       /* padding cells on each side of every dim */
+  int32_t reserved;
+} sth_array;
+
+typedef struct sth_stencil {
+  int32_t kind;            /* sth_shape (selects a build-time specialised kernel) */
+  int32_t radius;          /* the reference's radius type parameter R */
+  int32_t ndim;            /* stencil dims N <= array ndim; trailing dims pass through */
+  int32_t length;          /* L */
+  const int32_t* offsets;  /* host pointer, L * ndim ints, in fold order */
+  const void* weights;     /* host pointer to L weights (Kernel) or NULL */
+  int32_t weights_dtype;   /* sth_dtype of weights */
+  int32_t reserved;
+} sth_stencil;
+
+typedef struct sth_op {
+  int32_t kind;           /* sth_opkind */
+  uint32_t flags;         /* STH_OPF_* */
+  double alpha;           /* Laplacian coefficient */
+  uint64_t birth_mask;    /* Life: bit s => a dead cell with s live neighbours is born */
+  uint64_t survive_mask;  /* Life: bit s => a live cell with s live neighbours survives */
+} sth_op;
+
+/* Call flags. */
+#define STH_ASYNC 1u     /* return without synchronising the stream */
+#define STH_GRAPH 2u     /* sth_iterate: replay the steps as a captured hipGraph */
+#define STH_GENERIC 4u   /* force the runtime-offset kernel (testing / tuning) */
+
+typedef struct sth_ctx sth_ctx;
+
+/* Library version (STH_ABI_VERSION * 10000 + minor). */
+int sth_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* sth_last_error(void);
+
+/* Context: one per host thread.  `device` is a HIP device ordinal. */
+int sth_ctx_create(int device, sth_ctx** out);
+int sth_ctx_destroy(sth_ctx* ctx);
+
+/* Device memory owned by the library (the shim never frees it otherwise). */
+int sth_alloc(sth_ctx* ctx, size_t bytes, void** dptr);
+int sth_free(sth_ctx* ctx, void* dptr);
+/* kind: 0 host->device, 1 device->host, 2 device->device. */
+int sth_copy(sth_ctx* ctx, void* dst, const void* src, size_t bytes, int kind,
+             void* stream);
+int sth_synchronize(sth_ctx* ctx, void* stream);
+
+/*
+ * update_boundary! (src/array.jl:195-242): refresh the halo of a Halo-padded
+ * array from its interior.  `radius` is the halo width R.  No-op for USE.
+ */
+int sth_update_boundary(sth_ctx* ctx, const sth_array* a, int32_t radius,
+                        int32_t boundary, const void* padval, void* stream);
+
+/*
+ * gatherstencil!(f, dest, source) (src/gatherstencil.jl:89-103):
+ * dst[I] = op(stencil(src, I)) for every logical I.  For Halo padding the
+ * source halo is refreshed first, as update_boundary! does.  `padval`
+ * points to one value of src->dtype (REMOVE only).
+ */
+int sth_map(sth_ctx* ctx, const sth_array* src, const sth_array* dst,
+            const sth_stencil* st, int32_t boundary, const void* padval,
+            int32_t padding, const sth_op* op, uint32_t flags, void* stream);
+
+/*
+ * `steps` calls of mapstencil!(f, A::SwitchingStencilArray)
+ * (src/gatherstencil.jl:77-83 + switch, src/array.jl:610-611), ping-ponging
+ * buffers a (source) and b (dest) on device.  On return *current is 0 if
+ * `a` holds the newest state, 1 if `b` does (the switched wrapper's source).
+ * With STH_GRAPH the step pair is captured once into a hipGraph and replayed.
+ */
+int sth_iterate(sth_ctx* ctx, const sth_array* a, const sth_array* b,
+                const sth_stencil* st, int32_t boundary, const void* padval,
+                int32_t padding, const sth_op* op, int64_t steps,
+                uint32_t flags, int32_t* current, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* STENCILS_HIP_H */

@@ -1,0 +1,184 @@
+"""ORACLE — test infrastructure only.
+
+ctypes front end of oracle/liboracle.so, the plain-C restatement of
+Stencils.jl's gather path (see oracle/stencil_oracle.c for the reference
+file:line of each step).  Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg may import this module, and only as the checker or as the
+reported CPU baseline; the product never routes through it.
+
+All arrays here are host numpy arrays in Julia index order (dim 1 first).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+REPO = HERE.parent
+LIB_PATH = HERE / "liboracle.so"
+sys.path.insert(0, str(REPO / "stencils.jl_amd"))
+
+from stencils_hip import _abi  # noqa: E402  (descriptor structs only)
+
+MODE_NATIVE = 0
+MODE_F64 = 1
+
+_lib = None
+
+
+def build(force: bool = False):
+    """Compile liboracle.so with the Makefile beside this file."""
+    args = ["make", "-s", "-C", str(HERE)]
+    if force:
+        args.insert(2, "-B")
+    subprocess.run(args, check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = ctypes.CDLL(str(LIB_PATH))
+        P = ctypes.POINTER
+        L.orc_map.restype = ctypes.c_int
+        L.orc_map.argtypes = [P(_abi.SthArray), P(_abi.SthArray), ctypes.c_int, ctypes.c_int,
+                              P(ctypes.c_int32), ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                              P(_abi.SthOp), ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                              ctypes.c_int]
+        L.orc_iterate.restype = ctypes.c_int
+        L.orc_iterate.argtypes = [P(_abi.SthArray), P(_abi.SthArray), ctypes.c_int, ctypes.c_int,
+                                  P(ctypes.c_int32), ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                  P(_abi.SthOp), ctypes.c_void_p, ctypes.c_int, ctypes.c_int64,
+                                  ctypes.c_int, P(ctypes.c_int32)]
+        L.orc_update_boundary.restype = ctypes.c_int
+        L.orc_update_boundary.argtypes = [P(_abi.SthArray), ctypes.c_int, ctypes.c_void_p]
+        L.orc_out_dtype.restype = ctypes.c_int
+        L.orc_out_dtype.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint, ctypes.c_int,
+                                    ctypes.c_int]
+        for name in ("orc_f32_to_bf16", "orc_f32_to_f16"):
+            getattr(L, name).restype = ctypes.c_uint16
+            getattr(L, name).argtypes = [ctypes.c_float]
+        for name in ("orc_f16_to_f32", "orc_bf16_to_f32"):
+            getattr(L, name).restype = ctypes.c_float
+            getattr(L, name).argtypes = [ctypes.c_uint16]
+        _lib = L
+    return _lib
+
+
+def host_cores() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        return os.cpu_count() or 1
+
+
+def _desc(a: np.ndarray, dtype_code: int, logical_shape, halo: int) -> _abi.SthArray:
+    return _abi.make_array(a.ctypes.data, dtype_code, logical_shape,
+                           tuple(s // a.itemsize for s in a.strides), halo)
+
+
+def _op(kind, alpha=0.0, birth=(3,), survive=(2, 3), flags=0):
+    o = _abi.SthOp()
+    o.kind = kind
+    o.flags = flags
+    o.alpha = alpha
+    o.birth_mask = sum(1 << b for b in set(birth))
+    o.survive_mask = sum(1 << s for s in set(survive))
+    return o
+
+
+def map_raw(parent: np.ndarray, dtype_code: int, logical_shape, halo: int, offsets, st_ndim: int,
+            bc: int, padval_bytes, padding: int, op: _abi.SthOp, weights=None,
+            wdtype: int = -1, mode: int = MODE_NATIVE, nthreads: int = 1,
+            out_dtype: int | None = None) -> np.ndarray:
+    """One gatherstencil!(f, dest, source) on host arrays.  `parent` is the
+    source parent (halo included) and is refreshed in place for Halo padding,
+    as update_boundary! does.  Returns the result (logical shape)."""
+    L = lib()
+    offs = np.ascontiguousarray(np.asarray(offsets, dtype=np.int32).reshape(-1))
+    nL = offs.size // st_ndim
+    if out_dtype is None:
+        out_dtype = L.orc_out_dtype(dtype_code, op.kind, op.flags, wdtype, mode)
+        if out_dtype < 0:
+            raise ValueError("op / dtype combination not covered")
+    out = np.zeros(tuple(logical_shape), dtype=_abi.STH_TO_NUMPY[out_dtype], order="F")
+    src = _desc(parent, dtype_code, logical_shape, halo)
+    dst = _desc(out, out_dtype, logical_shape, 0)
+    pv = ctypes.create_string_buffer(padval_bytes, len(padval_bytes)) if padval_bytes else None
+    w = None if weights is None else np.ascontiguousarray(weights)
+    rc = L.orc_map(ctypes.byref(src), ctypes.byref(dst), nL, st_ndim,
+                   offs.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), bc, pv, padding,
+                   ctypes.byref(op), None if w is None else w.ctypes.data, wdtype, mode, nthreads)
+    if rc != 0:
+        raise ValueError(f"oracle status {rc}")
+    return out
+
+
+def iterate_raw(a: np.ndarray, b: np.ndarray, dtype_code: int, logical_shape, halo: int,
+                offsets, st_ndim: int, bc: int, padval_bytes, padding: int, op: _abi.SthOp,
+                steps: int, nthreads: int = 1) -> int:
+    """`steps` mapstencil! calls on the host buffers a (source) and b (dest);
+    returns 0 if `a` holds the newest state, 1 if `b` does."""
+    L = lib()
+    offs = np.ascontiguousarray(np.asarray(offsets, dtype=np.int32).reshape(-1))
+    nL = offs.size // st_ndim
+    da = _desc(a, dtype_code, logical_shape, halo)
+    db = _desc(b, dtype_code, logical_shape, halo)
+    pv = ctypes.create_string_buffer(padval_bytes, len(padval_bytes)) if padval_bytes else None
+    cur = ctypes.c_int32(0)
+    rc = L.orc_iterate(ctypes.byref(da), ctypes.byref(db), nL, st_ndim,
+                       offs.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), bc, pv, padding,
+                       ctypes.byref(op), None, -1, int(steps), nthreads, ctypes.byref(cur))
+    if rc != 0:
+        raise ValueError(f"oracle status {rc}")
+    return cur.value
+
+
+# ------------------------------------------------- front end over the mirror
+
+def _parts(sa):
+    """(dtype code, padval bytes, op flags) of a host-parent StencilArray."""
+    from stencils_hip.gather import _op_flags, _padval_bytes
+    dt = sa.sth_dtype()
+    return dt, _padval_bytes(sa), _op_flags(dt)
+
+
+def mapstencil(f, sa, mode: int = MODE_NATIVE, nthreads: int = 1) -> np.ndarray:
+    """Reference result of mapstencil(f, sa) for a host-parent StencilArray
+    (numpy parent, refreshed in place for Halo padding)."""
+    from stencils_hip.gather import _weights
+    from stencils_hip.ops import resolve
+    red = resolve(f)
+    dt, pv, flags = _parts(sa)
+    op = red.op(flags)
+    w, wdt = _weights(sa.stencil, dt)
+    if w is not None and mode == MODE_F64:
+        w = np.asarray(sa.stencil.weights, dtype=np.float64)
+        wdt = _abi.F64
+    return map_raw(sa.parent(), dt, sa.size(), sa.halo(), sa.stencil.offsets(), sa.stencil.N,
+                   sa.boundary.code, pv, sa.padding.code, op, w, wdt, mode, nthreads)
+
+
+def mapstencil_steps(f, A, steps: int, nthreads: int = 1):
+    """`steps` mapstencil!(f, A) on a host-parent SwitchingStencilArray, in place;
+    returns the switched array the last call would return."""
+    from stencils_hip.ops import resolve
+    red = resolve(f)
+    dt, pv, flags = _parts(A)
+    cur = iterate_raw(A.source, A.dest, dt, A.size(), A.halo(), A.stencil.offsets(), A.stencil.N,
+                      A.boundary.code, pv, A.padding.code, red.op(flags), steps, nthreads)
+    return A.switch() if cur == 1 else A
+
+
+def f32_to_bf16(x: float) -> int:
+    return lib().orc_f32_to_bf16(x)
+
+
+def bf16_to_f32(b: int) -> float:
+    return lib().orc_bf16_to_f32(b)

@@ -1,0 +1,590 @@
+// C ABI of libstencils_hip.so (include/stencils_hip.h): validation that
+// mirrors the reference's ArgumentErrors, kernel selection over the
+// catalogue, Halo refresh, and the ping-pong iteration engine with hipGraph
+// replay for SwitchingStencilArray (src/gatherstencil.jl:77-103).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdlib>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "halo_kernel.hpp"
+#include "life_kernel.hpp"
+#include "registry.hpp"
+#include "stencils_hip.h"
+
+namespace sth {
+
+std::vector<KernelEntry>& registry() {
+  static std::vector<KernelEntry> r;
+  return r;
+}
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define STH_HIP(call)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess) return fail(STH_EDEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+int dtype_size(int dt) {
+  switch (dt) {
+    case STH_BOOL: case STH_U8: case STH_I8: return 1;
+    case STH_U16: case STH_I16: case STH_F16: case STH_BF16: return 2;
+    case STH_U32: case STH_I32: case STH_F32: return 4;
+    case STH_U64: case STH_I64: case STH_F64: return 8;
+    default: return 0;
+  }
+}
+bool is_float(int dt) { return dt == STH_F16 || dt == STH_BF16 || dt == STH_F32 || dt == STH_F64; }
+bool is_signed(int dt) { return dt == STH_I8 || dt == STH_I16 || dt == STH_I32 || dt == STH_I64; }
+
+// Result type of `op` over `dt`, as the reference's _return_type gives it
+// (src/gatherstencil.jl:41-56); -1 if the GPU path does not cover it.
+int out_dtype(int dt, const sth_op& op, int wdt) {
+  const bool storage = dt == STH_BF16 || dt == STH_F16;
+  if (storage && !(op.flags & STH_OPF_ACC_F32)) return -1;
+  switch (op.kind) {
+    case STH_OP_SUM: return dt == STH_BOOL ? STH_I64 : dt;
+    case STH_OP_MEAN: return is_float(dt) ? dt : STH_F64;
+    case STH_OP_MIN: case STH_OP_MAX: return dt;
+    case STH_OP_KERNELPRODUCT:
+      if (dt == STH_BOOL) return -1;
+      if (storage) return wdt == STH_F32 ? dt : -1;
+      return wdt == dt ? dt : -1;
+    case STH_OP_LIFE: return is_float(dt) ? -1 : dt;
+    case STH_OP_LAPLACIAN: return is_float(dt) ? dt : -1;
+    default: return -1;
+  }
+}
+
+struct Plan {
+  MapParams p;
+  LaunchFn fn;
+  const char* kname;
+  int32_t halo;
+  int32_t pad_;
+  HaloParams hp;
+  void (*halo_fn)(const HaloParams&, hipStream_t);
+};
+
+DevArr make_devarr(const sth_array& a, bool with_halo) {
+  DevArr d{};
+  const int es = dtype_size(a.dtype);
+  const int h = with_halo ? a.halo : 0;
+  const int64_t h0 = h, h1 = a.ndim > 1 ? h : 0, h2 = a.ndim > 2 ? h : 0;
+  const int64_t s1 = a.ndim > 1 ? a.stride[1] : a.size[0];
+  const int64_t s2 = a.ndim > 2 ? a.stride[2] : s1 * (a.ndim > 1 ? a.size[1] + 2 * h1 : 1);
+  d.base = static_cast<char*>(a.data) + (h0 + h1 * s1 + h2 * s2) * es;
+  d.n0 = a.size[0];
+  d.n1 = a.ndim > 1 ? a.size[1] : 1;
+  d.n2 = a.ndim > 2 ? a.size[2] : 1;
+  d.s1 = s1;
+  d.s2 = s2;
+  d.h0 = static_cast<int32_t>(h0);
+  d.h1 = static_cast<int32_t>(h1);
+  d.h2 = static_cast<int32_t>(h2);
+  return d;
+}
+
+int check_array(const sth_array* a, const char* what) {
+  if (!a) return fail(STH_EINVAL, "%s descriptor is NULL", what);
+  if (a->ndim < 1 || a->ndim > 3) return fail(STH_EINVAL, "%s: ndim %d not in 1..3", what, a->ndim);
+  if (dtype_size(a->dtype) == 0) return fail(STH_EINVAL, "%s: unknown dtype %d", what, a->dtype);
+  if (!a->data) return fail(STH_EINVAL, "%s: data is NULL", what);
+  if (a->halo < 0) return fail(STH_EINVAL, "%s: negative halo", what);
+  for (int d = 0; d < a->ndim; ++d)
+    if (a->size[d] < 0) return fail(STH_EINVAL, "%s: negative size", what);
+  if (a->stride[0] != 1)
+    return fail(STH_ENOTSUP, "%s: dim 1 must be contiguous (stride 1), got %lld", what,
+                static_cast<long long>(a->stride[0]));
+  return STH_OK;
+}
+
+// Byte range [lo, hi) a descriptor covers (parent, halo included).
+void byte_range(const sth_array& a, uintptr_t* lo, uintptr_t* hi) {
+  const int es = dtype_size(a.dtype);
+  int64_t last = 0;
+  for (int d = 0; d < a.ndim; ++d) last += (a.size[d] + 2 * a.halo - 1) * a.stride[d];
+  *lo = reinterpret_cast<uintptr_t>(a.data);
+  *hi = *lo + static_cast<uintptr_t>((last + 1) * es);
+}
+
+bool offsets_equal(const int8_t* tbl, int L, const std::vector<int>& offs3) {
+  for (int k = 0; k < L; ++k)
+    for (int d = 0; d < 3; ++d)
+      if (tbl[3 * k + d] != offs3[3 * k + d]) return false;
+  return true;
+}
+
+int make_plan(const sth_array* src, const sth_array* dst, const sth_stencil* st, int32_t bc,
+              const void* padval, int32_t padding, const sth_op* op, uint32_t flags, Plan* out) {
+  int rc;
+  if ((rc = check_array(src, "source"))) return rc;
+  if ((rc = check_array(dst, "dest"))) return rc;
+  if (!st || !op) return fail(STH_EINVAL, "stencil or op descriptor is NULL");
+  if (src->ndim != dst->ndim) return fail(STH_EINVAL, "Source array sizes must match (ndims differ)");
+  for (int d = 0; d < src->ndim; ++d)
+    if (src->size[d] != dst->size[d])
+      return fail(STH_EINVAL, "Source array sizes must match. Found: dim %d %lld and %lld", d + 1,
+                  static_cast<long long>(src->size[d]), static_cast<long long>(dst->size[d]));
+  if (padding < STH_CONDITIONAL || padding > STH_HALO_IN) return fail(STH_EINVAL, "unknown padding %d", padding);
+  if (bc < STH_REMOVE || bc > STH_USE) return fail(STH_EINVAL, "unknown boundary %d", bc);
+  const bool halo_pad = padding != STH_CONDITIONAL;
+  if (!halo_pad && bc == STH_USE)
+    return fail(STH_EINVAL, "Use() boundary needs Halo padding (no getneighbor method for Conditional)");
+  if (!halo_pad && src->halo != 0) return fail(STH_EINVAL, "Conditional padding with a halo in the source descriptor");
+  if (bc == STH_REMOVE && !padval) return fail(STH_EINVAL, "Remove boundary without a padval");
+  if (st->length < 1 || !st->offsets) return fail(STH_EINVAL, "stencil has no offsets");
+  if (st->ndim < 1 || st->ndim > src->ndim)
+    return fail(STH_EINVAL, "stencil has %d dims, array %d", st->ndim, src->ndim);
+
+  // The reference checks R < every parent axis at construction (src/array.jl:451-453).
+  for (int d = 0; d < src->ndim; ++d) {
+    const int64_t ps = src->size[d] + 2 * static_cast<int64_t>(src->halo);
+    if (!(std::abs(st->radius) < ps))
+      return fail(STH_EINVAL, "stencil radius is larger than array axis %lld", static_cast<long long>(ps));
+  }
+  if (halo_pad && src->halo != st->radius)
+    return fail(STH_EINVAL, "Halo padding: halo width %d differs from the stencil radius %d", src->halo,
+                st->radius);
+  if (halo_pad && (bc == STH_WRAP || bc == STH_REFLECT))
+    for (int d = 0; d < src->ndim; ++d)
+      if (src->halo >= src->size[d])
+        return fail(STH_EINVAL, "BoundsError: halo %d >= axis %lld under Wrap/Reflect", src->halo,
+                    static_cast<long long>(src->size[d]));
+
+  const int sdt = src->dtype;
+  const int wdt = st->weights ? st->weights_dtype : -1;
+  if (op->kind == STH_OP_KERNELPRODUCT && !st->weights)
+    return fail(STH_EINVAL, "kernelproduct needs a Kernel stencil (weights)");
+  const int odt = out_dtype(sdt, *op, wdt);
+  if (odt < 0) return fail(STH_ENOTSUP, "op %d over dtype %d (weights %d) is not accelerated", op->kind, sdt, wdt);
+  if (odt != dst->dtype)
+    return fail(STH_EINVAL, "dest dtype %d does not match the result type %d", dst->dtype, odt);
+
+  // offsets, zero-extended to 3 dims; reach per dim
+  const int L = st->length;
+  std::vector<int> offs3(3 * L, 0);
+  int reach[3] = {0, 0, 0};
+  for (int k = 0; k < L; ++k)
+    for (int d = 0; d < st->ndim; ++d) {
+      const int v = st->offsets[k * st->ndim + d];
+      offs3[3 * k + d] = v;
+      reach[d] = std::max(reach[d], std::abs(v));
+    }
+  const int64_t n[3] = {src->size[0], src->ndim > 1 ? src->size[1] : 1, src->ndim > 2 ? src->size[2] : 1};
+  for (int d = 0; d < 3; ++d) {
+    if (halo_pad && d < src->ndim && reach[d] > src->halo)
+      return fail(STH_ENOTSUP, "stencil reach %d exceeds the halo %d (reference reads out of bounds)", reach[d],
+                  src->halo);
+    if (!halo_pad && reach[d] >= n[d] && reach[d] > 0)
+      return fail(STH_ENOTSUP, "stencil reach %d >= axis %lld (reference reads out of bounds)", reach[d],
+                  static_cast<long long>(n[d]));
+    if (reach[d] > 127) return fail(STH_ENOTSUP, "stencil reach %d too large", reach[d]);
+  }
+
+  // aliasing
+  {
+    uintptr_t a0, a1, b0, b1;
+    byte_range(*src, &a0, &a1);
+    byte_range(*dst, &b0, &b1);
+    if (a0 < b1 && b0 < a1) return fail(STH_EINVAL, "dest overlaps source");
+  }
+
+  // Zero every byte (padding included): plans are compared bytewise as graph keys.
+  Plan& pl = *out;
+  std::memset(static_cast<void*>(&pl), 0, sizeof(Plan));
+  MapParams& p = pl.p;
+  p.src = make_devarr(*src, halo_pad);
+  p.dst = make_devarr(*dst, true);
+  p.bc = bc;
+  p.read_halo = halo_pad ? 1 : 0;
+  p.rx = reach[0];
+  p.ry = reach[1];
+  p.rz = reach[2];
+  p.L = L;
+  p.padval = 0;
+  if (bc == STH_REMOVE) std::memcpy(&p.padval, padval, dtype_size(sdt));
+  p.op.alpha = op->alpha;
+  p.op.birth = op->birth_mask;
+  p.op.survive = op->survive_mask;
+  p.xcd_remap = 1;
+
+  if (op->kind == STH_OP_KERNELPRODUCT) {
+    if (L > kMaxRuntimeL) return fail(STH_ENOTSUP, "kernel with %d weights (max %d)", L, kMaxRuntimeL);
+    const char* w = static_cast<const char*>(st->weights);
+    for (int k = 0; k < L; ++k) {
+      if (sdt == STH_F64) {
+        std::memcpy(&p.w.d[k], w + 8 * k, 8);
+      } else if (is_float(sdt)) {
+        std::memcpy(&p.w.f[k], w + 4 * k, 4);
+      } else {
+        const int es = dtype_size(wdt);
+        uint64_t u = 0;
+        std::memcpy(&u, w + es * k, es);
+        if (is_signed(wdt) && es < 8) {
+          const int sh = 64 - 8 * es;
+          u = static_cast<uint64_t>(static_cast<int64_t>(u << sh) >> sh);
+        }
+        p.w.u64[k] = u;
+      }
+    }
+  }
+
+  // ---- kernel choice ------------------------------------------------
+  const bool moore1 = st->ndim == 2 && L == 8 && [&] {
+    static const int m[8][2] = {{-1, -1}, {0, -1}, {1, -1}, {-1, 0}, {1, 0}, {-1, 1}, {0, 1}, {1, 1}};
+    for (int k = 0; k < 8; ++k)
+      if (offs3[3 * k] != m[k][0] || offs3[3 * k + 1] != m[k][1]) return false;
+    return true;
+  }();
+  const auto aligned16 = [](const DevArr& d) {
+    return (reinterpret_cast<uintptr_t>(d.base) % 16 == 0) && (d.s1 % 16 == 0);
+  };
+  if (!(flags & STH_GENERIC) && op->kind == STH_OP_LIFE && (sdt == STH_U8 || sdt == STH_BOOL) &&
+      moore1 && src->ndim == 2 && n[0] % 16 == 0 && aligned16(p.src) && aligned16(p.dst)) {
+    constexpr int KY = 16;
+    p.tiles_x = static_cast<int32_t>((n[0] + 1023) / 1024);
+    p.tiles_y = static_cast<int32_t>((n[1] + 4 * KY - 1) / (4 * KY));
+    p.tiles_z = 1;
+    pl.fn = &launch_life<KY>;
+    pl.kname = "life_swar";
+    if (halo_pad && bc != STH_USE) {
+      // map boundaries in registers and refresh the source halo in the same launch
+      p.read_halo = 0;
+      p.fuse_halo = 1;
+    }
+  } else {
+    const KernelEntry* pick = nullptr;
+    if (!(flags & STH_GENERIC)) {
+      for (const auto& e : registry()) {
+        if (e.L == 0 || e.dtype != sdt || e.op != op->kind || e.L != L || e.ndim != st->ndim) continue;
+        if (offsets_equal(e.offs, L, offs3)) {
+          pick = &e;
+          break;
+        }
+      }
+    }
+    if (!pick) {
+      if (L > kMaxRuntimeL) return fail(STH_ENOTSUP, "stencil with %d offsets (runtime max %d)", L, kMaxRuntimeL);
+      const int cfg = reach[2] > 0 ? CFG_RT3D : (n[1] == 1 && reach[1] == 0 ? CFG_RT1D : CFG_RT2D);
+      for (const auto& e : registry())
+        if (e.L == 0 && e.dtype == sdt && e.op == op->kind && e.cfg == cfg) {
+          pick = &e;
+          break;
+        }
+      for (int k = 0; k < L; ++k)
+        p.offs[k] = (offs3[3 * k] & 0xff) | ((offs3[3 * k + 1] & 0xff) << 8) | ((offs3[3 * k + 2] & 0xff) << 16);
+    }
+    if (!pick) return fail(STH_ENOTSUP, "no kernel for dtype %d op %d", sdt, op->kind);
+    const TileDims td = tile_dims(pick->cfg);
+    const int64_t lds = static_cast<int64_t>(td.tx + 2 * reach[0]) * (td.ty + 2 * reach[1]) *
+                        (td.tz + 2 * reach[2]) * dtype_size(sdt);
+    if (lds > 65536) return fail(STH_ENOTSUP, "stencil reach needs %lld bytes of LDS per tile", static_cast<long long>(lds));
+    const int64_t tx = (n[0] + td.tx - 1) / td.tx, ty = (n[1] + td.ty - 1) / td.ty,
+                  tz = (n[2] + td.tz - 1) / td.tz;
+    if (tx * ty * tz > (int64_t(1) << 31) - 1) return fail(STH_ENOTSUP, "array too large for one launch");
+    p.tiles_x = static_cast<int32_t>(tx);
+    p.tiles_y = static_cast<int32_t>(ty);
+    p.tiles_z = static_cast<int32_t>(tz);
+    pl.fn = pick->fn;
+    pl.kname = pick->name;
+  }
+  if (n[0] * n[1] * n[2] == 0) pl.fn = nullptr;  // empty array: nothing to launch
+
+  // ---- halo refresh (update_boundary!) --------------------------------
+  if (halo_pad && bc != STH_USE && src->halo > 0 && !p.fuse_halo) {
+    pl.halo = 1;
+    HaloParams& h = pl.hp;
+    h.a = make_devarr(*src, true);
+    h.ndim = src->ndim;
+    h.R = src->halo;
+    h.bc = bc;
+    h.padval = p.padval;
+    for (int d = 0; d < 3; ++d) {
+      int64_t cells = 0;
+      if (d < src->ndim) {
+        cells = 2 * static_cast<int64_t>(h.R);
+        for (int e = 0; e < 3; ++e)
+          if (e != d) cells *= n[e] + (e < src->ndim ? 2 * h.R : 0);
+      }
+      h.cells[d] = cells;
+      h.blocks[d] = (cells + 255) / 256;
+    }
+    switch (dtype_size(sdt)) {
+      case 1: pl.halo_fn = &launch_halo<uint8_t>; break;
+      case 2: pl.halo_fn = &launch_halo<uint16_t>; break;
+      case 4: pl.halo_fn = &launch_halo<uint32_t>; break;
+      default: pl.halo_fn = &launch_halo<uint64_t>; break;
+    }
+  }
+  return STH_OK;
+}
+
+int run_plan(const Plan& pl, hipStream_t s) {
+  if (pl.halo) pl.halo_fn(pl.hp, s);
+  if (pl.fn) pl.fn(pl.p, s);
+  STH_HIP(hipGetLastError());
+  return STH_OK;
+}
+
+}  // namespace
+}  // namespace sth
+
+struct sth_ctx {
+  int device = 0;
+  hipStream_t capture = nullptr;  // private stream used only for graph capture
+  std::mutex mu;
+  std::map<std::string, hipGraphExec_t> graphs;
+};
+
+using namespace sth;
+
+#define STH_GUARD_BEGIN try {
+#define STH_GUARD_END                                                   \
+  }                                                                     \
+  catch (const std::exception& e) {                                     \
+    return fail(STH_EDEVICE, "internal error: %s", e.what());           \
+  }                                                                     \
+  catch (...) {                                                         \
+    return fail(STH_EDEVICE, "internal error");                         \
+  }
+
+namespace {
+int set_device(sth_ctx* ctx) {
+  if (!ctx) return fail(STH_EINVAL, "context is NULL");
+  STH_HIP(hipSetDevice(ctx->device));
+  return STH_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int sth_version(void) { return STH_ABI_VERSION * 10000 + 0; }
+
+const char* sth_last_error(void) { return g_err.c_str(); }
+
+int sth_ctx_create(int device, sth_ctx** out) {
+  STH_GUARD_BEGIN
+  if (!out) return fail(STH_EINVAL, "out is NULL");
+  int n = 0;
+  STH_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(STH_EINVAL, "device %d not in 0..%d", device, n - 1);
+  STH_HIP(hipSetDevice(device));
+  auto* c = new sth_ctx();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->capture, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(STH_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+  }
+  *out = c;
+  g_err.clear();
+  return STH_OK;
+  STH_GUARD_END
+}
+
+int sth_ctx_destroy(sth_ctx* ctx) {
+  STH_GUARD_BEGIN
+  if (!ctx) return STH_OK;
+  (void)hipSetDevice(ctx->device);
+  for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+  if (ctx->capture) (void)hipStreamDestroy(ctx->capture);
+  delete ctx;
+  return STH_OK;
+  STH_GUARD_END
+}
+
+int sth_alloc(sth_ctx* ctx, size_t bytes, void** dptr) {
+  STH_GUARD_BEGIN
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  if (!dptr) return fail(STH_EINVAL, "dptr is NULL");
+  hipError_t e = hipMalloc(dptr, bytes ? bytes : 1);
+  if (e == hipErrorOutOfMemory) return fail(STH_ENOMEM, "hipMalloc(%zu): out of device memory", bytes);
+  if (e != hipSuccess) return fail(STH_EDEVICE, "hipMalloc: %s", hipGetErrorString(e));
+  return STH_OK;
+  STH_GUARD_END
+}
+
+int sth_free(sth_ctx* ctx, void* dptr) {
+  STH_GUARD_BEGIN
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  if (dptr) STH_HIP(hipFree(dptr));
+  return STH_OK;
+  STH_GUARD_END
+}
+
+int sth_copy(sth_ctx* ctx, void* dst, const void* src, size_t bytes, int kind, void* stream) {
+  STH_GUARD_BEGIN
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  if (kind < 0 || kind > 2) return fail(STH_EINVAL, "copy kind %d not in 0..2", kind);
+  auto s = static_cast<hipStream_t>(stream);
+  STH_HIP(hipMemcpyAsync(dst, src, bytes, k, s));
+  STH_HIP(hipStreamSynchronize(s));
+  return STH_OK;
+  STH_GUARD_END
+}
+
+int sth_synchronize(sth_ctx* ctx, void* stream) {
+  STH_GUARD_BEGIN
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  STH_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  return STH_OK;
+  STH_GUARD_END
+}
+
+int sth_update_boundary(sth_ctx* ctx, const sth_array* a, int32_t radius, int32_t boundary,
+                        const void* padval, void* stream) {
+  STH_GUARD_BEGIN
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  if ((rc = check_array(a, "array"))) return rc;
+  if (radius != a->halo) return fail(STH_EINVAL, "radius %d differs from the halo %d", radius, a->halo);
+  if (boundary < STH_REMOVE || boundary > STH_USE) return fail(STH_EINVAL, "unknown boundary %d", boundary);
+  if (boundary == STH_USE || radius == 0) return STH_OK;
+  if (boundary == STH_REMOVE && !padval) return fail(STH_EINVAL, "Remove boundary without a padval");
+  if (boundary == STH_WRAP || boundary == STH_REFLECT)
+    for (int d = 0; d < a->ndim; ++d)
+      if (radius >= a->size[d])
+        return fail(STH_EINVAL, "BoundsError: halo %d >= axis %lld under Wrap/Reflect", radius,
+                    static_cast<long long>(a->size[d]));
+  HaloParams h{};
+  h.a = make_devarr(*a, true);
+  h.ndim = a->ndim;
+  h.R = radius;
+  h.bc = boundary;
+  if (boundary == STH_REMOVE) std::memcpy(&h.padval, padval, dtype_size(a->dtype));
+  const int64_t n[3] = {h.a.n0, h.a.n1, h.a.n2};
+  for (int d = 0; d < 3; ++d) {
+    int64_t cells = 0;
+    if (d < a->ndim) {
+      cells = 2 * static_cast<int64_t>(radius);
+      for (int e = 0; e < 3; ++e)
+        if (e != d) cells *= n[e] + (e < a->ndim ? 2 * radius : 0);
+    }
+    h.cells[d] = cells;
+    h.blocks[d] = (cells + 255) / 256;
+  }
+  auto s = static_cast<hipStream_t>(stream);
+  switch (dtype_size(a->dtype)) {
+    case 1: launch_halo<uint8_t>(h, s); break;
+    case 2: launch_halo<uint16_t>(h, s); break;
+    case 4: launch_halo<uint32_t>(h, s); break;
+    default: launch_halo<uint64_t>(h, s); break;
+  }
+  STH_HIP(hipGetLastError());
+  STH_HIP(hipStreamSynchronize(s));
+  return STH_OK;
+  STH_GUARD_END
+}
+
+int sth_map(sth_ctx* ctx, const sth_array* src, const sth_array* dst, const sth_stencil* st,
+            int32_t boundary, const void* padval, int32_t padding, const sth_op* op, uint32_t flags,
+            void* stream) {
+  STH_GUARD_BEGIN
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  Plan pl;
+  if ((rc = make_plan(src, dst, st, boundary, padval, padding, op, flags, &pl))) return rc;
+  auto s = static_cast<hipStream_t>(stream);
+  if ((rc = run_plan(pl, s))) return rc;
+  if (!(flags & STH_ASYNC)) STH_HIP(hipStreamSynchronize(s));
+  return STH_OK;
+  STH_GUARD_END
+}
+
+int sth_iterate(sth_ctx* ctx, const sth_array* a, const sth_array* b, const sth_stencil* st,
+                int32_t boundary, const void* padval, int32_t padding, const sth_op* op, int64_t steps,
+                uint32_t flags, int32_t* current, void* stream) {
+  STH_GUARD_BEGIN
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  if (steps < 0) return fail(STH_EINVAL, "negative step count");
+  if (!current) return fail(STH_EINVAL, "current is NULL");
+  if (!a || !b) return fail(STH_EINVAL, "buffer descriptor is NULL");
+  if (a->dtype != b->dtype || a->halo != b->halo)
+    return fail(STH_EINVAL, "source and dest buffers differ in type or halo");
+  if (op && op->kind == STH_OP_MEAN && !is_float(a->dtype))
+    return fail(STH_EINVAL, "mapstencil! cannot store a Float64 mean into an integer buffer");
+  Plan ab, ba;
+  if ((rc = make_plan(a, b, st, boundary, padval, padding, op, flags, &ab))) return rc;
+  if ((rc = make_plan(b, a, st, boundary, padval, padding, op, flags, &ba))) return rc;
+  auto s = static_cast<hipStream_t>(stream);
+  *current = static_cast<int32_t>(steps % 2);
+  if (!(flags & STH_GRAPH)) {
+    for (int64_t i = 0; i < steps; ++i)
+      if ((rc = run_plan(i % 2 ? ba : ab, s))) return rc;
+  } else {
+    // A graph of G = 2U steps (U pairs A->B, B->A) is captured once per
+    // parameter set and replayed; the remainder uses a second cached graph.
+    const int64_t U = steps >= 64 ? 16 : std::max<int64_t>(1, steps / 2);
+    const int64_t G = 2 * U;
+    auto get_graph = [&](int64_t nsteps, hipGraphExec_t* exec) -> int {
+      std::string key(reinterpret_cast<const char*>(&ab), sizeof(Plan));
+      key.append(reinterpret_cast<const char*>(&ba), sizeof(Plan));
+      key.append(reinterpret_cast<const char*>(&nsteps), sizeof nsteps);
+      std::lock_guard<std::mutex> lk(ctx->mu);
+      auto it = ctx->graphs.find(key);
+      if (it != ctx->graphs.end()) {
+        *exec = it->second;
+        return STH_OK;
+      }
+      STH_HIP(hipStreamBeginCapture(ctx->capture, hipStreamCaptureModeThreadLocal));
+      int r2 = STH_OK;
+      for (int64_t i = 0; i < nsteps && r2 == STH_OK; ++i) r2 = run_plan(i % 2 ? ba : ab, ctx->capture);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(ctx->capture, &g);
+      if (r2) return r2;
+      if (e != hipSuccess) return fail(STH_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+      hipGraphExec_t x = nullptr;
+      e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) return fail(STH_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+      ctx->graphs.emplace(key, x);
+      *exec = x;
+      return STH_OK;
+    };
+    const int64_t reps = steps / G, rem = steps % G;
+    if (reps > 0) {
+      hipGraphExec_t x;
+      if ((rc = get_graph(G, &x))) return rc;
+      for (int64_t i = 0; i < reps; ++i) STH_HIP(hipGraphLaunch(x, s));
+    }
+    if (rem > 0) {
+      hipGraphExec_t x;
+      if ((rc = get_graph(rem, &x))) return rc;
+      STH_HIP(hipGraphLaunch(x, s));
+    }
+  }
+  STH_HIP(hipGetLastError());
+  if (!(flags & STH_ASYNC)) STH_HIP(hipStreamSynchronize(s));
+  return STH_OK;
+  STH_GUARD_END
+}
+
+}  // extern "C"

@@ -1,0 +1,269 @@
+// Tiled gather kernel: dst[I] = op(neighbours of src around I), one template
+// for 1-D, 2-D and 3-D arrays.
+//
+// Per workgroup (256 threads = 4 wave64s):
+//   1. stage the (TX+2rx) x (TY+2ry) x (TZ+2rz) source box around the output
+//      tile in LDS.  Boundary conditions are resolved here, per staged
+//      element, and only in blocks that touch an edge (a uniform branch), so
+//      the compute loop below never branches on boundaries.
+//   2. each thread owns one x column of the tile and KR consecutive rows; it
+//      folds the stencil over LDS in offset order (the reference's left fold,
+//      src/gatherstencil.jl:105-109 + StaticArrays reductions).  For shapes
+//      fixed at build time (`Sh::kStatic`) the offsets are constants, the
+//      loops unroll and identical LDS reads across the thread's rows are
+//      shared in registers.
+//   3. results are written with coalesced stores along dim 1.
+#pragma once
+
+#include "common.hpp"
+
+namespace sth {
+
+// Arithmetic type of the fold for (storage type, op).
+template <typename S, int OPK>
+struct AccOf {
+  using type = S;
+};
+template <int OPK> struct AccOf<bf16_t, OPK> { using type = float; };
+template <int OPK> struct AccOf<f16_t, OPK> { using type = float; };
+#define STH_INT_ACC(T)                                                        \
+  template <> struct AccOf<T, OP_SUM> { using type = Elem<T>::uacc; };        \
+  template <> struct AccOf<T, OP_KP> { using type = Elem<T>::uacc; };         \
+  template <> struct AccOf<T, OP_MEAN> { using type = double; };              \
+  template <> struct AccOf<T, OP_LIFE> { using type = uint32_t; };
+STH_INT_ACC(bool)
+STH_INT_ACC(uint8_t)
+STH_INT_ACC(int8_t)
+STH_INT_ACC(uint16_t)
+STH_INT_ACC(int16_t)
+STH_INT_ACC(uint32_t)
+STH_INT_ACC(int32_t)
+STH_INT_ACC(uint64_t)
+STH_INT_ACC(int64_t)
+#undef STH_INT_ACC
+
+// Storage type of the result (the reference's _return_type).
+template <typename S, int OPK>
+struct OutOf {
+  using type = S;
+};
+template <> struct OutOf<bool, OP_SUM> { using type = int64_t; };
+#define STH_INT_MEAN(T) \
+  template <> struct OutOf<T, OP_MEAN> { using type = double; };
+STH_INT_MEAN(bool)
+STH_INT_MEAN(uint8_t)
+STH_INT_MEAN(int8_t)
+STH_INT_MEAN(uint16_t)
+STH_INT_MEAN(int16_t)
+STH_INT_MEAN(uint32_t)
+STH_INT_MEAN(int32_t)
+STH_INT_MEAN(uint64_t)
+STH_INT_MEAN(int64_t)
+#undef STH_INT_MEAN
+
+// A stencil whose offsets arrive at run time (Positional, NamedStencil and
+// shapes outside the build-time catalogue).
+struct RuntimeShape {
+  static constexpr bool kStatic = false;
+  static constexpr int L = 0, RX = 0, RY = 0, RZ = 0;
+};
+
+template <typename A>
+__device__ __forceinline__ A weight(const MapParams& p, int k) {
+  if constexpr (__is_same(A, double)) {
+    return p.w.d[k];
+  } else if constexpr (__is_same(A, float)) {
+    return p.w.f[k];
+  } else {
+    return static_cast<A>(p.w.u64[k]);
+  }
+}
+
+template <typename A>
+__device__ __forceinline__ A fold_min(A a, A b) {
+  if constexpr (__is_same(A, float) || __is_same(A, double)) return jl_min(a, b);
+  else return b < a ? b : a;
+}
+template <typename A>
+__device__ __forceinline__ A fold_max(A a, A b) {
+  if constexpr (__is_same(A, float) || __is_same(A, double)) return jl_max(a, b);
+  else return b > a ? b : a;
+}
+
+// One output value from the staged box; `c` is the LDS index of the centre.
+template <typename S, int OPK, class Sh, int W, int HW>
+__device__ __forceinline__ typename AccOf<S, OPK>::type reduce_cell(
+    const typename Elem<S>::raw* __restrict__ tile, int c, const MapParams& p,
+    int wdyn, int hwdyn) {
+  using A = typename AccOf<S, OPK>::type;
+  using Raw = typename Elem<S>::raw;
+  const int L = Sh::kStatic ? Sh::L : p.L;
+  A acc = A(0);
+  uint32_t cnt = 0;
+  auto step = [&](int k, int lin) {
+    const Raw rv = tile[c + lin];
+    if constexpr (OPK == OP_LIFE) {
+      cnt += nonzero(rv) ? 1u : 0u;
+    } else {
+      const A v = widen<A, S>(rv);
+      if constexpr (OPK == OP_SUM || OPK == OP_MEAN || OPK == OP_LAP) {
+        acc = k == 0 ? v : acc + v;
+      } else if constexpr (OPK == OP_MIN) {
+        acc = k == 0 ? v : fold_min(acc, v);
+      } else if constexpr (OPK == OP_MAX) {
+        acc = k == 0 ? v : fold_max(acc, v);
+      } else if constexpr (OPK == OP_KP) {
+        const A pr = v * weight<A>(p, k);
+        acc = acc + pr;
+      }
+    }
+  };
+  if constexpr (Sh::kStatic) {
+#pragma unroll
+    for (int k = 0; k < Sh::L; ++k)
+      step(k, Sh::off[k][2] * HW + Sh::off[k][1] * W + Sh::off[k][0]);
+  } else {
+    for (int k = 0; k < L; ++k) {
+      const int o = p.offs[k];
+      const int dx = static_cast<int8_t>(o & 0xff);
+      const int dy = static_cast<int8_t>((o >> 8) & 0xff);
+      const int dz = static_cast<int8_t>((o >> 16) & 0xff);
+      step(k, dz * hwdyn + dy * wdyn + dx);
+    }
+  }
+  if constexpr (OPK == OP_MEAN) {
+    acc = acc / static_cast<A>(L);
+  } else if constexpr (OPK == OP_LAP) {
+    const A cv = widen<A, S>(tile[c]);
+    const A t = static_cast<A>(L) * cv;
+    const A d = acc - t;
+    const A a = static_cast<A>(p.op.alpha) * d;
+    acc = cv + a;
+  } else if constexpr (OPK == OP_LIFE) {
+    const bool alive = nonzero(tile[c]);
+    const uint64_t mask = alive ? p.op.survive : p.op.birth;
+    acc = cnt < 64u ? static_cast<A>((mask >> cnt) & 1u) : A(0);
+  }
+  return acc;
+}
+
+template <typename Raw>
+__device__ __forceinline__ Raw load_raw(const DevArr& a, int64_t x, int64_t y, int64_t z) {
+  const Raw* p = reinterpret_cast<const Raw*>(a.base);
+  return p[z * a.s2 + y * a.s1 + x];
+}
+
+template <typename S, int OPK, class Sh, int TX, int TY, int TZ>
+__global__ __launch_bounds__(256) void tile_kernel(MapParams p) {
+  using Raw = typename Elem<S>::raw;
+  using O = typename OutOf<S, OPK>::type;
+  using ORaw = typename Elem<O>::raw;
+  constexpr int kThreads = 256;
+  static_assert(kThreads % TX == 0, "TX must divide the block");
+  constexpr int G = kThreads / TX;
+  static_assert((TY * TZ) % G == 0, "rows must split evenly over thread groups");
+  constexpr int KR = TY * TZ / G;
+
+  const int rx = Sh::kStatic ? Sh::RX : p.rx;
+  const int ry = Sh::kStatic ? Sh::RY : p.ry;
+  const int rz = Sh::kStatic ? Sh::RZ : p.rz;
+  constexpr int WS = TX + 2 * Sh::RX, HS = TY + 2 * Sh::RY, DS = TZ + 2 * Sh::RZ;
+  const int W = Sh::kStatic ? WS : TX + 2 * rx;
+  const int H = Sh::kStatic ? HS : TY + 2 * ry;
+  const int D = Sh::kStatic ? DS : TZ + 2 * rz;
+  const int HW = H * W;
+
+  extern __shared__ __align__(16) unsigned char smem_dyn[];
+  Raw* tile = reinterpret_cast<Raw*>(smem_dyn);
+
+  const int nblocks = p.tiles_x * p.tiles_y * p.tiles_z;
+  const int bid = xcd_block_id(static_cast<int>(blockIdx.x), nblocks, p.xcd_remap);
+  const int bx = bid % p.tiles_x;
+  const int byz = bid / p.tiles_x;
+  const int by = byz % p.tiles_y;
+  const int bz = byz / p.tiles_y;
+
+  const int64_t n0 = p.src.n0, n1 = p.src.n1, n2 = p.src.n2;
+  const int64_t X0 = static_cast<int64_t>(bx) * TX - rx;
+  const int64_t Y0 = static_cast<int64_t>(by) * TY - ry;
+  const int64_t Z0 = static_cast<int64_t>(bz) * TZ - rz;
+
+  // ---- 1. stage the source box -------------------------------------
+  // Memory that may be read directly: the logical array, plus the halo when
+  // the padding keeps a valid one.
+  const int total = HW * D;
+  const int64_t h0 = p.read_halo ? p.src.h0 : 0;
+  const int64_t h1 = p.read_halo ? p.src.h1 : 0;
+  const int64_t h2 = p.read_halo ? p.src.h2 : 0;
+  const bool direct = X0 >= -h0 && X0 + W <= n0 + h0 && Y0 >= -h1 && Y0 + H <= n1 + h1 &&
+                      Z0 >= -h2 && Z0 + D <= n2 + h2;
+  if (direct) {
+    for (int idx = threadIdx.x; idx < total; idx += kThreads) {
+      const int lz = idx / HW;
+      const int r = idx - lz * HW;
+      const int ly = r / W;
+      const int lx = r - ly * W;
+      tile[idx] = load_raw<Raw>(p.src, X0 + lx, Y0 + ly, Z0 + lz);
+    }
+  } else {
+    const Raw pv = raw_from_bits<Raw>(p.padval);
+    for (int idx = threadIdx.x; idx < total; idx += kThreads) {
+      const int lz = idx / HW;
+      const int r = idx - lz * HW;
+      const int ly = r / W;
+      const int lx = r - ly * W;
+      const int64_t gx = X0 + lx, gy = Y0 + ly, gz = Z0 + lz;
+      Raw v = Raw(0);
+      // Only the box [-r, n + r) feeds valid outputs; the rest of a partial
+      // tile is never read by a stored result.
+      const bool needed = gx >= -rx && gx < n0 + rx && gy >= -ry && gy < n1 + ry &&
+                          gz >= -rz && gz < n2 + rz;
+      if (needed) {
+        const bool inb = gx >= 0 && gx < n0 && gy >= 0 && gy < n1 && gz >= 0 && gz < n2;
+        if (inb || p.read_halo) {
+          v = load_raw<Raw>(p.src, gx, gy, gz);
+        } else if (p.bc == STH_REMOVE) {
+          v = pv;
+        } else {
+          v = load_raw<Raw>(p.src, map_index(gx, n0, p.bc), map_index(gy, n1, p.bc),
+                            map_index(gz, n2, p.bc));
+        }
+      }
+      tile[idx] = v;
+    }
+  }
+  __syncthreads();
+
+  // ---- 2. fold -------------------------------------------------------
+  const int lx = threadIdx.x % TX;
+  const int grp = threadIdx.x / TX;
+  const int64_t gx = static_cast<int64_t>(bx) * TX + lx;
+  ORaw* dstp = reinterpret_cast<ORaw*>(p.dst.base);
+#pragma unroll
+  for (int j = 0; j < KR; ++j) {
+    const int row = grp * KR + j;
+    const int ly = row % TY;
+    const int lz = row / TY;
+    const int c = (lz + rz) * HW + (ly + ry) * W + (lx + rx);
+    const auto acc = reduce_cell<S, OPK, Sh, WS, WS * HS>(tile, c, p, W, HW);
+    const int64_t gy = static_cast<int64_t>(by) * TY + ly;
+    const int64_t gz = static_cast<int64_t>(bz) * TZ + lz;
+    if (gx < n0 && gy < n1 && gz < n2)
+      dstp[gz * p.dst.s2 + gy * p.dst.s1 + gx] = narrow<O>(acc);
+  }
+}
+
+// Host-side launcher with a uniform signature for the kernel registry.
+template <typename S, int OPK, class Sh, int TX, int TY, int TZ>
+void launch_tile(const MapParams& p, hipStream_t stream) {
+  using Raw = typename Elem<S>::raw;
+  const int rx = Sh::kStatic ? Sh::RX : p.rx;
+  const int ry = Sh::kStatic ? Sh::RY : p.ry;
+  const int rz = Sh::kStatic ? Sh::RZ : p.rz;
+  const size_t lds = static_cast<size_t>(TX + 2 * rx) * (TY + 2 * ry) * (TZ + 2 * rz) * sizeof(Raw);
+  const unsigned nblocks = static_cast<unsigned>(p.tiles_x) * p.tiles_y * p.tiles_z;
+  hipLaunchKernelGGL((tile_kernel<S, OPK, Sh, TX, TY, TZ>), dim3(nblocks), dim3(256), lds,
+                     stream, p);
+}
+
+}  // namespace sth

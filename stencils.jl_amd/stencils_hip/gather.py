@@ -1,0 +1,295 @@
+"""mapstencil / mapstencil_ (= gatherstencil / gatherstencil!) over the C ABI.
+
+Mirrors src/gatherstencil.jl:
+  gatherstencil(f, A::StencilArray)                :14-21
+  gatherstencil(f, hood, A; boundary, padding)     :22-39
+  gatherstencil!(f, A::SwitchingStencilArray)      :77-83
+  gatherstencil!(f, dest, source)                  :89-103
+  mapstencil / mapstencil! deprecated aliases      :127-128
+Python has no `!`, so the mutating forms end in an underscore.
+
+Device parents (torch tensors) are used in place.  Host parents (numpy) are
+copied to the device for the call and the results copied back, including
+the Halo refresh the reference performs on the source (update_boundary!).
+There is no CPU path: without the HIP library or a device the call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _abi, _lib
+from .array import (AbstractStencilArray, StencilArray, SwitchingStencilArray,
+                    colmajor_copy, colmajor_empty, is_device, sth_dtype_of,
+                    torch_dtype, to_numpy)
+from .boundary import Conditional, Halo, Remove, Use
+from .ops import Reduction, Unsupported, resolve
+from .stencil import Kernel, Stencil
+
+
+def _torch():
+    import torch
+    return torch
+
+
+# --------------------------------------------------------------- descriptors
+
+class _Call:
+    """Keeps ctypes buffers alive for the duration of one library call."""
+
+    def __init__(self):
+        self.keep = []
+
+    def hold(self, x):
+        self.keep.append(x)
+        return x
+
+
+def _padval_bytes(sa: AbstractStencilArray):
+    bc = sa.boundary
+    if not isinstance(bc, Remove):
+        return None
+    pv = bc.padval
+    if pv is None:
+        raise Unsupported("Remove() with padval=nothing cannot be folded (reference MethodError)")
+    dt = sa.sth_dtype()
+    if isinstance(pv, np.generic) and _abi.NUMPY_TO_STH.get(np.dtype(type(pv))) != dt:
+        # The reference promotes the stencil element type (src/gatherstencil.jl:50).
+        raise Unsupported(f"padval of type {type(pv).__name__} promotes the element type")
+    if dt == _abi.BF16 or dt == _abi.F16:
+        torch = _torch()
+        t = torch.tensor([float(pv)], dtype=torch_dtype(dt))
+        if float(t[0]) != float(pv):
+            raise Unsupported("padval is not representable in the storage type")
+        return t.view(torch.int16).numpy().tobytes()
+    npdt = _abi.STH_TO_NUMPY[dt]
+    v = np.array(pv).astype(npdt)
+    if not (v == np.array(pv)).all() or (dt not in _abi.FLOAT_DTYPES and isinstance(pv, float)):
+        raise Unsupported("padval is not exactly representable in the element type")
+    return v.tobytes()
+
+
+def _weights(st: Stencil, src_dt: int):
+    if not isinstance(st, Kernel):
+        return None, -1
+    w = np.asarray(st.weights)
+    if src_dt in (_abi.BF16, _abi.F16):
+        return np.ascontiguousarray(w, dtype=np.float32), _abi.F32
+    wdt = _abi.NUMPY_TO_STH.get(w.dtype, -1)
+    if w.dtype.kind == "i" and src_dt in _abi.FLOAT_DTYPES:
+        # integer weights promote to the data's float type without loss here
+        w = w.astype(_abi.STH_TO_NUMPY[src_dt])
+        wdt = src_dt
+    return np.ascontiguousarray(w), wdt
+
+
+def stencil_descriptor(st: Stencil, src_dt: int, call: _Call) -> _abi.SthStencil:
+    if not st.gpu_supported():
+        raise Unsupported(f"{st!r} is not supported on the GPU path")
+    offs = np.ascontiguousarray(np.array(st.offsets(), dtype=np.int32).reshape(-1))
+    call.hold(offs)
+    d = _abi.SthStencil()
+    d.kind = st.kind
+    d.radius = st.radius()
+    d.ndim = st.N
+    d.length = len(st)
+    d.offsets = offs.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    w, wdt = _weights(st, src_dt)
+    if w is not None:
+        call.hold(w)
+        d.weights = w.ctypes.data
+        d.weights_dtype = wdt
+    else:
+        d.weights = None
+        d.weights_dtype = -1
+    return d
+
+
+def _op_flags(src_dt: int) -> int:
+    return _abi.OPF_ACC_F32 if src_dt in (_abi.BF16, _abi.F16) else 0
+
+
+def result_dtype(f, sa: AbstractStencilArray) -> int:
+    red = resolve(f)
+    dt = sa.sth_dtype()
+    _, wdt = _weights(sa.stencil, dt)
+    o = _abi.out_dtype(dt, red.kind, _op_flags(dt), wdt)
+    if o < 0:
+        raise Unsupported(f"{red!r} over {_abi.DTYPE_NAMES[dt]} is not accelerated")
+    return o
+
+
+def _device_index(x) -> int:
+    return x.device.index if x.device.index is not None else _torch().cuda.current_device()
+
+
+def _stream(dev: int):
+    return ctypes.c_void_p(_torch().cuda.current_stream(dev).cuda_stream)
+
+
+def _desc(x, shape, halo) -> _abi.SthArray:
+    from .array import data_ptr, elem_strides
+    return _abi.make_array(data_ptr(x), sth_dtype_of(x), shape, elem_strides(x), halo)
+
+
+# ---------------------------------------------------------------- the calls
+
+def _map_device(f, dst, dst_halo, sa: AbstractStencilArray, src_buf, flags: int = 0):
+    """One sth_map call on device buffers `src_buf` (parent) -> `dst`."""
+    red = resolve(f)
+    dt = sa.sth_dtype()
+    call = _Call()
+    st = stencil_descriptor(sa.stencil, dt, call)
+    op = red.op(_op_flags(dt))
+    pv = _padval_bytes(sa)
+    pvbuf = call.hold(ctypes.create_string_buffer(pv, len(pv))) if pv is not None else None
+    size = sa.size()
+    src = _desc(src_buf, size, sa.halo())
+    dsc = _desc(dst, size, dst_halo)
+    dev = _device_index(src_buf)
+    L = _lib.lib()
+    ctx = _lib.context(dev)
+    _lib.check(L.sth_map(ctx, ctypes.byref(src), ctypes.byref(dsc), ctypes.byref(st),
+                         sa.boundary.code, pvbuf, sa.padding.code, ctypes.byref(op),
+                         flags, _stream(dev)))
+
+
+def _require_backend():
+    torch = _torch()
+    if not torch.cuda.is_available():
+        raise _lib.BackendUnavailable("no HIP device is visible; the GPU path has no CPU fallback")
+
+
+def gatherstencil_(f, dest, source=None, *args, flags: int = 0):
+    """gatherstencil!(f, dest, source) or gatherstencil!(f, A::SwitchingStencilArray)."""
+    if isinstance(dest, SwitchingStencilArray) and source is None:
+        return _switching_step(f, dest, flags)
+    if args:
+        raise Unsupported("multi-array gather (extra arrays) is not accelerated")
+    if not isinstance(source, AbstractStencilArray):
+        raise TypeError("source must be a StencilArray")
+    resolve(f)
+    _require_backend()
+    if isinstance(dest, AbstractStencilArray):
+        # a StencilArray destination is written through its halo offset
+        # (setindex! with add_halo, src/array.jl:355-362)
+        if tuple(dest.size()) != tuple(source.size()):
+            raise ValueError(f"Source array sizes must match. Found: {tuple(dest.size())} and "
+                             f"{tuple(source.size())}")
+        if not (is_device(dest.parent()) and is_device(source.parent())):
+            raise Unsupported("StencilArray destinations must live on the device")
+        _map_device(f, dest.parent(), dest.halo(), source, source.parent(), flags)
+        return dest
+    if tuple(np.shape(dest)) != tuple(source.size()):
+        raise ValueError(f"Source array sizes must match. Found: {tuple(np.shape(dest))} and "
+                         f"{tuple(source.size())}")
+    src_parent = source.parent()
+    halo_refresh = isinstance(source.padding, Halo) and not isinstance(source.boundary, Use)
+    if is_device(src_parent) and is_device(dest):
+        _map_device(f, dest, 0, source, src_parent, flags)
+        return dest
+    # host round trip
+    torch = _torch()
+    dev = torch.cuda.current_device()
+    sbuf = src_parent if is_device(src_parent) else colmajor_copy(src_parent, dev)
+    dbuf = colmajor_empty(np.shape(dest), torch_dtype(sth_dtype_of(dest)), dev)
+    _map_device(f, dbuf, 0, source, sbuf, flags)
+    if is_device(dest):
+        dest.copy_(dbuf)
+    else:
+        dest[...] = to_numpy(dbuf)
+    if halo_refresh and not is_device(src_parent):
+        src_parent[...] = to_numpy(sbuf)
+    return dest
+
+
+def _alloc_like(parent, dt_code: int, shape):
+    if is_device(parent):
+        return colmajor_empty(shape, torch_dtype(dt_code), parent.device)
+    return np.zeros(shape, dtype=_abi.STH_TO_NUMPY[dt_code], order="F")
+
+
+def gatherstencil(f, *args, boundary=None, padding=None):
+    """gatherstencil(f, A::StencilArray) or gatherstencil(f, hood, A; boundary, padding)."""
+    if not args:
+        raise TypeError("gatherstencil needs an array")
+    if isinstance(args[0], Stencil):
+        hood, A = args[0], args[1]
+        rest = args[2:]
+        sa = StencilArray(A, hood, boundary=boundary, padding=padding)
+        T = result_dtype(f, sa)
+        size = sa.size()
+        dest = _alloc_like(A, T, size)
+        return gatherstencil_(f, dest, sa, *rest)
+    sa = args[0]
+    if not isinstance(sa, AbstractStencilArray):
+        raise TypeError("gatherstencil(f, A) needs a StencilArray")
+    T = result_dtype(f, sa)
+    dest = _alloc_like(sa.parent(), T, sa.size())
+    return gatherstencil_(f, dest, sa, *args[1:])
+
+
+def _inner(sa, buf):
+    h = sa.halo()
+    return buf[tuple(slice(h, s - h) for s in buf.shape)]
+
+
+def _switching_step(f, A: SwitchingStencilArray, flags: int = 0):
+    """gatherstencil!(f, A::SwitchingStencilArray): dest <- f(source), then switch."""
+    resolve(f)
+    _require_backend()
+    if result_dtype(f, A) != A.sth_dtype():
+        raise ValueError("the result type differs from the array's element type")
+    if is_device(A.source):
+        _map_device(f, A.dest, A.halo(), A, A.source, flags)
+        return A.switch()
+    torch = _torch()
+    dev = torch.cuda.current_device()
+    s = colmajor_copy(A.source, dev)
+    d = colmajor_copy(A.dest, dev)
+    _map_device(f, d, A.halo(), A, s, flags)
+    A.source[...] = to_numpy(s)
+    A.dest[...] = to_numpy(d)
+    return A.switch()
+
+
+def mapstencil_steps(f, A: SwitchingStencilArray, steps: int, graph: bool = True,
+                     flags: int = 0):
+    """`steps` calls of mapstencil!(f, A) on device in one library call
+    (sth_iterate); with `graph` the steps replay as a captured hipGraph.
+    Returns the array switched as the last call of the loop would return it."""
+    red = resolve(f)
+    _require_backend()
+    if result_dtype(f, A) != A.sth_dtype():
+        raise ValueError("the result type differs from the array's element type")
+    torch = _torch()
+    host = not is_device(A.source)
+    dev = torch.cuda.current_device() if host else _device_index(A.source)
+    a = colmajor_copy(A.source, dev) if host else A.source
+    b = colmajor_copy(A.dest, dev) if host else A.dest
+    dt = A.sth_dtype()
+    call = _Call()
+    st = stencil_descriptor(A.stencil, dt, call)
+    op = red.op(_op_flags(dt))
+    pv = _padval_bytes(A)
+    pvbuf = call.hold(ctypes.create_string_buffer(pv, len(pv))) if pv is not None else None
+    size = A.size()
+    da = _desc(a, size, A.halo())
+    db = _desc(b, size, A.halo())
+    cur = ctypes.c_int32(0)
+    L = _lib.lib()
+    ctx = _lib.context(dev)
+    fl = flags | (_abi.GRAPH if graph else 0)
+    _lib.check(L.sth_iterate(ctx, ctypes.byref(da), ctypes.byref(db), ctypes.byref(st),
+                             A.boundary.code, pvbuf, A.padding.code, ctypes.byref(op),
+                             int(steps), fl, ctypes.byref(cur), _stream(dev)))
+    if host:
+        A.source[...] = to_numpy(a)
+        A.dest[...] = to_numpy(b)
+    return A.switch() if cur.value == 1 else A
+
+
+# Deprecated aliases of the reference (src/gatherstencil.jl:127-128).
+mapstencil = gatherstencil
+mapstencil_ = gatherstencil_

@@ -1,0 +1,99 @@
+"""The fixed reduction set the GPU path accelerates.
+
+``sum``, ``mean``, ``minimum``, ``maximum`` and ``kernelproduct`` are the
+functions a Stencils.jl user passes to ``mapstencil`` (StaticArrays /
+Statistics reductions and src/stencils/kernel.jl:34-50).  ``LifeRule`` and
+``Laplacian`` are canonical callables the reference does not define: their
+CPU semantics are stated here and restated by the oracle, so every path
+evaluates the same formula.  Any other function is not accelerated
+(``resolve`` raises ``Unsupported``).
+"""
+from __future__ import annotations
+
+import builtins
+import statistics
+
+import numpy as np
+
+from . import _abi
+
+
+class Unsupported(NotImplementedError):
+    """The GPU path does not cover this call (the Julia shim falls back to the
+    reference method in this case)."""
+
+
+class Reduction:
+    def __init__(self, name: str, kind: int):
+        self.name = name
+        self.kind = kind
+
+    def __repr__(self):
+        return self.name
+
+    def op(self, flags: int = 0) -> _abi.SthOp:
+        o = _abi.SthOp()
+        o.kind = self.kind
+        o.flags = flags
+        return o
+
+
+sum = Reduction("sum", _abi.OP_SUM)
+mean = Reduction("mean", _abi.OP_MEAN)
+minimum = Reduction("minimum", _abi.OP_MIN)
+maximum = Reduction("maximum", _abi.OP_MAX)
+kernelproduct = Reduction("kernelproduct", _abi.OP_KERNELPRODUCT)
+
+
+class LifeRule(Reduction):
+    """Life-like automaton: with ``s`` the number of nonzero neighbours and
+    ``c`` the centre, the new cell is ``T(s in birth)`` if ``c == 0`` and
+    ``T(s in survive)`` otherwise.  Conway's Life is B3/S23, the default.
+    For 0/1 data this is ``s = sum(neighbors(hood))``."""
+
+    def __init__(self, birth=(3,), survive=(2, 3)):
+        super().__init__(f"LifeRule(B{''.join(map(str, birth))}/S{''.join(map(str, survive))})",
+                         _abi.OP_LIFE)
+        self.birth = tuple(int(b) for b in birth)
+        self.survive = tuple(int(s) for s in survive)
+        if any(not 0 <= v < 64 for v in self.birth + self.survive):
+            raise ValueError("neighbour counts must be in 0..63")
+
+    def op(self, flags: int = 0) -> _abi.SthOp:
+        o = super().op(flags)
+        o.birth_mask = builtins.sum(1 << b for b in set(self.birth))
+        o.survive_mask = builtins.sum(1 << s for s in set(self.survive))
+        return o
+
+
+class Laplacian(Reduction):
+    """Explicit diffusion step ``c + alpha * (sum(neighbors) - L * c)``,
+    evaluated in that order in the element type (L = stencil length, 2N for
+    VonNeumann(1, N))."""
+
+    def __init__(self, alpha: float):
+        super().__init__(f"Laplacian({alpha})", _abi.OP_LAPLACIAN)
+        self.alpha = float(alpha)
+
+    def op(self, flags: int = 0) -> _abi.SthOp:
+        o = super().op(flags)
+        o.alpha = self.alpha
+        return o
+
+
+_ALIASES = {
+    id(builtins.sum): sum, id(np.sum): sum,
+    id(np.mean): mean, id(statistics.mean): mean, id(statistics.fmean): mean,
+    id(builtins.min): minimum, id(np.min): minimum, id(np.amin): minimum,
+    id(builtins.max): maximum, id(np.max): maximum, id(np.amax): maximum,
+}
+
+
+def resolve(f) -> Reduction:
+    """Map a user function onto the accelerated set, or raise Unsupported."""
+    if isinstance(f, Reduction):
+        return f
+    r = _ALIASES.get(id(f))
+    if r is None:
+        raise Unsupported(f"function {f!r} is not in the accelerated reduction set")
+    return r

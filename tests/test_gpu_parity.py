@@ -1,0 +1,259 @@
+"""HIP path vs the oracle, through the C ABI (libstencils_hip.so).
+
+Integer, Bool and Float64/Float32 results must be bit-identical to the oracle
+(the reference's fold order, no contraction).  BF16/F16 storage results must
+be bit-identical to the oracle's f32-accumulate restatement; their error
+against the FP64 oracle is checked separately.
+"""
+import numpy as np
+import pytest
+
+import stencils_hip as sh
+from conftest import GOLDEN
+from _spec import F, make_boundary, make_input, make_padding, make_stencil
+
+pytestmark = pytest.mark.gpu
+
+
+def _np_input(shape, dtype, seed):
+    rng = np.random.default_rng(seed)
+    if dtype == np.bool_:
+        return np.asfortranarray(rng.random(shape) < 0.4)
+    if np.dtype(dtype).kind in "iu":
+        info = np.iinfo(dtype)
+        lo, hi = max(info.min, -50), min(info.max, 50)
+        return np.asfortranarray(rng.integers(lo, hi + 1, shape).astype(dtype))
+    return np.asfortranarray((rng.random(shape) * 2 - 1).astype(dtype))
+
+
+def _both(f, A, st, bc, pad, oracle):
+    """(GPU result, oracle result, GPU source parent, oracle source parent)."""
+    ga = sh.StencilArray(A.copy(order="F"), st, boundary=bc, padding=pad)
+    oa = sh.StencilArray(A.copy(order="F"), st, boundary=bc, padding=pad)
+    g = sh.mapstencil(f, ga)
+    o = oracle.mapstencil(f, oa)
+    return g, o, ga.parent(), oa.parent()
+
+
+# ------------------------------------------------------------ goldens
+
+def _cases():
+    for c in GOLDEN["cases"]:
+        for pad in c["paddings"]:
+            yield pytest.param(c, pad, id=f"{c['name']}-{pad}")
+
+
+@pytest.mark.parametrize("case,pad", list(_cases()))
+def test_golden_cases_on_gpu(gpu, case, pad):
+    A = make_input(case)
+    st = make_stencil(case["stencil"])
+    sa = sh.StencilArray(A.copy(order="F"), st, boundary=make_boundary(case["boundary"], A.dtype),
+                         padding=make_padding(pad))
+    out = sh.mapstencil(F[case["f"]], sa)
+    exp = np.asarray(case["expected"])
+    if case.get("interior"):
+        out = out[1:-1, 1:-1]
+    if case["tol"] is None:
+        np.testing.assert_array_equal(out, exp)
+    else:
+        np.testing.assert_allclose(out, exp, rtol=0, atol=case["tol"])
+
+
+# ------------------------------------------------- random parity sweep
+
+SHAPES_2D = [sh.Window(1), sh.Window(2), sh.Window(3), sh.Moore(1), sh.Moore(2),
+             sh.VonNeumann(1), sh.VonNeumann(2), sh.VonNeumann(3), sh.Circle(1), sh.Circle(2),
+             sh.Circle(3), sh.Positional((2, -1), (0, 0), (-1, 3), (1, 1)), sh.Cross(2),
+             sh.Annulus(3), sh.Rectangle((-2, 1), (0, 2)), sh.Ordinal(2)]
+BOUNDS = {"remove": lambda dt: sh.Remove(np.dtype(dt).type(0) if np.dtype(dt) != np.bool_ else np.bool_(False)),
+          "remove7": lambda dt: sh.Remove(np.dtype(dt).type(7) if np.dtype(dt).kind in "iuf" else np.bool_(True)),
+          "wrap": lambda dt: sh.Wrap(), "reflect": lambda dt: sh.Reflect()}
+PADS = {"cond": sh.Conditional(), "halo_out": sh.Halo("out"), "halo_in": sh.Halo("in")}
+
+
+@pytest.mark.parametrize("pad", list(PADS))
+@pytest.mark.parametrize("bname", list(BOUNDS))
+@pytest.mark.parametrize("st", SHAPES_2D, ids=repr)
+def test_parity_f64_2d(gpu, oracle, st, bname, pad):
+    A = _np_input((70, 45), np.float64, 1)
+    bc = BOUNDS[bname](np.float64)
+    for f in (sh.sum, sh.mean, sh.minimum, sh.maximum):
+        g, o, gp, op_ = _both(f, A, st, bc, PADS[pad], oracle)
+        np.testing.assert_array_equal(g, o, err_msg=f"{f}")
+        np.testing.assert_array_equal(gp, op_)  # Halo refresh side effect
+    w = np.random.default_rng(4).standard_normal(len(st))
+    K = sh.Kernel(st, w)
+    g, o, _, _ = _both(sh.kernelproduct, A, K, bc, PADS[pad], oracle)
+    np.testing.assert_array_equal(g, o)
+
+
+@pytest.mark.parametrize("bname", ["remove", "wrap", "reflect"])
+@pytest.mark.parametrize("st", [sh.Window(1), sh.Circle(3), sh.VonNeumann(2), sh.Window(3),
+                                sh.Positional((1, -2), (0, 0), (-3, 1))], ids=repr)
+def test_parity_f32_2d(gpu, oracle, st, bname):
+    A = _np_input((131, 67), np.float32, 2)
+    bc = BOUNDS[bname](np.float32)
+    for pad in ("cond", "halo_out"):
+        for f in (sh.sum, sh.mean, sh.minimum, sh.maximum):
+            g, o, _, _ = _both(f, A, st, bc, PADS[pad], oracle)
+            np.testing.assert_array_equal(g, o)
+        K = sh.Kernel(st, np.random.default_rng(5).standard_normal(len(st)).astype(np.float32))
+        g, o, _, _ = _both(sh.kernelproduct, A, K, bc, PADS[pad], oracle)
+        np.testing.assert_array_equal(g, o)
+
+
+@pytest.mark.parametrize("dt", [np.bool_, np.uint8, np.int8, np.uint16, np.int16, np.uint32,
+                                np.int32, np.uint64, np.int64], ids=lambda d: np.dtype(d).name)
+def test_parity_integers(gpu, oracle, dt):
+    A = _np_input((53, 38), dt, 3)
+    for st in (sh.Window(1), sh.Moore(2), sh.Circle(2), sh.Positional((0, 1), (2, -1))):
+        for bname in ("remove7", "wrap", "reflect"):
+            bc = BOUNDS[bname](dt)
+            for pad in ("cond", "halo_out"):
+                ops = [sh.sum, sh.mean, sh.minimum, sh.maximum, sh.LifeRule(), sh.LifeRule((1, 3), (0, 5))]
+                for f in ops:
+                    g, o, gp, op_ = _both(f, A, st, bc, PADS[pad], oracle)
+                    np.testing.assert_array_equal(g, o, err_msg=f"{st} {bname} {pad} {f}")
+                    np.testing.assert_array_equal(gp, op_)
+                if dt != np.bool_:
+                    K = sh.Kernel(st, np.arange(1, len(st) + 1).astype(dt))
+                    g, o, _, _ = _both(sh.kernelproduct, A, K, bc, PADS[pad], oracle)
+                    np.testing.assert_array_equal(g, o)
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 17, 300, 1000])
+def test_parity_1d(gpu, oracle, n):
+    A = _np_input((n,), np.float64, 6)
+    for st in (sh.Window(1, 1), sh.Moore(1, 1), sh.Positional((-1,), (0,), (2,))):
+        if max(st.reach()) >= n:
+            continue
+        for bname in ("remove7", "wrap", "reflect"):
+            for pad in ("cond", "halo_out", "halo_in"):
+                if pad == "halo_in" and n <= 2 * st.radius():
+                    continue
+                g, o, gp, op_ = _both(sh.mean, A, st, BOUNDS[bname](np.float64), PADS[pad], oracle)
+                np.testing.assert_array_equal(g, o)
+                np.testing.assert_array_equal(gp, op_)
+
+
+@pytest.mark.parametrize("st", [sh.VonNeumann(1, 3), sh.Moore(1, 3), sh.Circle(2, 3),
+                                sh.Window(1, 2), sh.Positional((1, 0, -1), (0, 0, 0), (0, -2, 1))],
+                         ids=repr)
+def test_parity_3d(gpu, oracle, st):
+    A = _np_input((21, 18, 17), np.float64, 7)
+    for bname in ("remove7", "wrap", "reflect"):
+        for pad in ("cond", "halo_out"):
+            for f in (sh.sum, sh.mean, sh.maximum):
+                g, o, _, _ = _both(f, A, st, BOUNDS[bname](np.float64), PADS[pad], oracle)
+                np.testing.assert_array_equal(g, o)
+            if st.N == 3 and st.offsets() == sh.VonNeumann(1, 3).offsets():
+                g, o, _, _ = _both(sh.Laplacian(0.15), A, st, BOUNDS[bname](np.float64), PADS[pad], oracle)
+                np.testing.assert_array_equal(g, o)
+
+
+@pytest.mark.parametrize("tname", ["bfloat16", "float16"])
+def test_parity_storage_types(gpu, oracle, tname):
+    """BF16/F16 storage, f32 accumulation, one rounding at the store: bit-exact
+    against the oracle's restatement of the same arithmetic."""
+    import torch
+    tdt = getattr(torch, tname)
+    A32 = _np_input((96, 77), np.float32, 8)
+    At = torch.from_numpy(np.ascontiguousarray(A32)).to(tdt)
+    raw = At.view(torch.int16).numpy().view(np.uint16)
+    code = sh._abi.BF16 if tname == "bfloat16" else sh._abi.F16
+    dev = torch.device("cuda")
+    for st in (sh.Window(3), sh.VonNeumann(1), sh.Circle(3), sh.Cross(1)):
+        for bname, bc in (("wrap", sh.Wrap()), ("reflect", sh.Reflect()), ("remove", sh.Remove(0.0))):
+            for f in (sh.sum, sh.mean, sh.maximum, sh.Laplacian(0.2)):
+                if isinstance(f, sh.Laplacian) and st.offsets() != sh.VonNeumann(1).offsets():
+                    continue
+                src = sh.colmajor_empty(A32.shape, tdt, dev)
+                src.copy_(At.to(dev))
+                g = sh.mapstencil(f, sh.StencilArray(src, st, boundary=bc))
+                g = g.cpu().view(torch.int16).numpy().view(np.uint16)
+                pv = b"\0\0" if bname == "remove" else None
+                o = oracle.map_raw(np.asfortranarray(raw), code, A32.shape, 0, st.offsets(), 2,
+                                   bc.code, pv, 0, f.op(sh._abi.OPF_ACC_F32))
+                np.testing.assert_array_equal(g, o.view(np.uint16), err_msg=f"{st} {bname} {f}")
+            K = sh.Kernel(st, np.random.default_rng(1).random(len(st)).astype(np.float32))
+            src = sh.colmajor_empty(A32.shape, tdt, dev)
+            src.copy_(At.to(dev))
+            g = sh.mapstencil(sh.kernelproduct, sh.StencilArray(src, K, boundary=bc))
+            g = g.cpu().view(torch.int16).numpy().view(np.uint16)
+            pv = b"\0\0" if bname == "remove" else None
+            o = oracle.map_raw(np.asfortranarray(raw), code, A32.shape, 0, st.offsets(), 2, bc.code,
+                               pv, 0, sh.kernelproduct.op(sh._abi.OPF_ACC_F32), K.weights, sh._abi.F32)
+            np.testing.assert_array_equal(g, o.view(np.uint16))
+
+
+# ---------------------------------------------------- iteration / Life
+
+@pytest.mark.parametrize("pad", ["cond", "halo_out", "halo_in"])
+@pytest.mark.parametrize("bname", ["remove", "remove7", "wrap", "reflect"])
+@pytest.mark.parametrize("shape", [(256, 96), (48, 40), (2048, 64)])
+def test_life_iterate(gpu, oracle, shape, bname, pad):
+    A = (np.random.default_rng(9).random(shape) < 0.3).astype(np.uint8)
+    bc = BOUNDS[bname](np.uint8)
+    for rule in (sh.LifeRule(), sh.LifeRule((3, 6), (2, 3))):
+        for graph in (False, True):
+            G = sh.SwitchingStencilArray(np.asfortranarray(A), sh.Moore(1), boundary=bc, padding=PADS[pad])
+            O = sh.SwitchingStencilArray(np.asfortranarray(A), sh.Moore(1), boundary=bc, padding=PADS[pad])
+            G = sh.mapstencil_steps(rule, G, 9, graph=graph)
+            O = oracle.mapstencil_steps(rule, O, 9)
+            np.testing.assert_array_equal(G.source, O.source)
+            np.testing.assert_array_equal(G.dest, O.dest)
+
+
+def test_life_device_resident_swar_path(gpu, oracle):
+    """Halo{:out} device buffers take the SWAR kernel; compare with the oracle."""
+    import torch
+    n = 1024
+    A = (np.random.default_rng(10).random((n, n)) < 0.3).astype(np.uint8)
+    dev = torch.device("cuda")
+    t = sh.colmajor_empty((n, n), torch.uint8, dev)
+    t.copy_(torch.from_numpy(np.ascontiguousarray(A)).to(dev))
+    G = sh.SwitchingStencilArray(t, sh.Moore(1), boundary=sh.Wrap(), padding=sh.Halo())
+    G = sh.mapstencil_steps(sh.LifeRule(), G, 33, graph=True)
+    O = sh.SwitchingStencilArray(np.asfortranarray(A), sh.Moore(1), boundary=sh.Wrap(), padding=sh.Halo())
+    O = oracle.mapstencil_steps(sh.LifeRule(), O, 33, nthreads=8)
+    np.testing.assert_array_equal(G.source.cpu().numpy(), O.source)
+    np.testing.assert_array_equal(G.dest.cpu().numpy(), O.dest)
+
+
+def test_single_steps_match_iterate(gpu):
+    A = np.random.default_rng(12).random((64, 48))
+    S1 = sh.SwitchingStencilArray(A.copy(order="F"), sh.Window(1), boundary=sh.Reflect())
+    for _ in range(5):
+        S1 = sh.mapstencil_(sh.mean, S1)
+    S2 = sh.SwitchingStencilArray(A.copy(order="F"), sh.Window(1), boundary=sh.Reflect())
+    S2 = sh.mapstencil_steps(sh.mean, S2, 5, graph=True)
+    np.testing.assert_array_equal(np.asarray(S1), np.asarray(S2))
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_heat_3d_iterate(gpu, oracle, dt):
+    A = _np_input((40, 33, 29), dt, 13)
+    for pad in ("cond", "halo_out"):
+        G = sh.SwitchingStencilArray(A.copy(order="F"), sh.VonNeumann(1, 3), boundary=sh.Wrap(),
+                                     padding=PADS[pad])
+        O = sh.SwitchingStencilArray(A.copy(order="F"), sh.VonNeumann(1, 3), boundary=sh.Wrap(),
+                                     padding=PADS[pad])
+        G = sh.mapstencil_steps(sh.Laplacian(0.15), G, 21, graph=True)
+        O = oracle.mapstencil_steps(sh.Laplacian(0.15), O, 21)
+        np.testing.assert_array_equal(np.asarray(G), np.asarray(O))
+
+
+# ------------------------------------------------------------ errors
+
+def test_argument_errors(gpu):
+    A = np.zeros((10, 10))
+    with pytest.raises(ValueError, match="sizes must match"):
+        sh.mapstencil_(sh.mean, np.zeros((9, 10)), sh.StencilArray(A, sh.Window(1)))
+    with pytest.raises(ValueError, match="Use"):
+        sh.mapstencil(sh.mean, sh.StencilArray(A, sh.Window(1), boundary=sh.Use()))
+    with pytest.raises(sh.Unsupported):  # reach 3 > halo 1 (Positional signed radius)
+        sh.mapstencil(sh.sum, sh.StencilArray(A, sh.Positional((-3, 0), (1, 0)), padding=sh.Halo()))
+    with pytest.raises(sh.Unsupported):
+        sh.mapstencil(lambda s: s, sh.StencilArray(A, sh.Window(1)))
+    with pytest.raises(sh.Unsupported):
+        sh.mapstencil(sh.mean, sh.StencilArray(np.zeros((5, 5, 5)), sh.Window(1, 3)))
