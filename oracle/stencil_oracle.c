@@ -436,6 +436,10 @@ int orc_out_dtype(int src_dt, int opkind, unsigned flags, int wdtype, int mode) 
 int orc_update_boundary(const sth_array* a, int bc, const void* padval) {
   if (bc == STH_USE || a->halo == 0) return STH_OK;
   const int R = a->halo, nd = a->ndim, es = dtype_size(a->dtype);
+  /* halo_val maps -R..-1 into the interior: a BoundsError when R >= axis */
+  if (bc != STH_REMOVE)
+    for (int d = 0; d < nd; ++d)
+      if (R >= a->size[d]) return STH_EINVAL;
   int64_t ps[3] = {1, 1, 1};
   for (int d = 0; d < nd; ++d) ps[d] = a->size[d] + 2 * R;
   char* base = (char*)a->data;

@@ -89,7 +89,7 @@ def test_parity_f64_2d(gpu, oracle, st, bname, pad):
 
 @pytest.mark.parametrize("bname", ["remove", "wrap", "reflect"])
 @pytest.mark.parametrize("st", [sh.Window(1), sh.Circle(3), sh.VonNeumann(2), sh.Window(3),
-                                sh.Positional((1, -2), (0, 0), (-3, 1))], ids=repr)
+                                sh.Positional((3, -2), (0, 0), (-3, 1))], ids=repr)
 def test_parity_f32_2d(gpu, oracle, st, bname):
     A = _np_input((131, 67), np.float32, 2)
     bc = BOUNDS[bname](np.float32)
@@ -129,15 +129,16 @@ def test_parity_1d(gpu, oracle, n):
             continue
         for bname in ("remove7", "wrap", "reflect"):
             for pad in ("cond", "halo_out", "halo_in"):
-                if pad == "halo_in" and n <= 2 * st.radius():
-                    continue
+                logical = n - 2 * st.radius() if pad == "halo_in" else n
+                if logical < 1 or (pad != "cond" and bname != "remove7" and st.radius() >= logical):
+                    continue  # the reference throws a BoundsError here
                 g, o, gp, op_ = _both(sh.mean, A, st, BOUNDS[bname](np.float64), PADS[pad], oracle)
                 np.testing.assert_array_equal(g, o)
                 np.testing.assert_array_equal(gp, op_)
 
 
 @pytest.mark.parametrize("st", [sh.VonNeumann(1, 3), sh.Moore(1, 3), sh.Circle(2, 3),
-                                sh.Window(1, 2), sh.Positional((1, 0, -1), (0, 0, 0), (0, -2, 1))],
+                                sh.Window(1, 2), sh.Positional((2, 0, -1), (0, 0, 0), (0, -2, 1))],
                          ids=repr)
 def test_parity_3d(gpu, oracle, st):
     A = _np_input((21, 18, 17), np.float64, 7)
