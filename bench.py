@@ -103,6 +103,20 @@ def main():
     bytes_per_step = 2 * cells  # 1 B read + 1 B written per interior cell
     achieved = bytes_per_step / per_step_ev / 1e9
 
+    # measured stream-copy bandwidth on the same bytes (one read + one write)
+    src_t = torch.empty(cells, dtype=torch.uint8, device="cuda").fill_(1)
+    dst_t = torch.empty_like(src_t)
+    for _ in range(5):
+        dst_t.copy_(src_t)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(50):
+        dst_t.copy_(src_t)
+    e1.record()
+    torch.cuda.synchronize()
+    copy_gbs = 2 * cells / (e0.elapsed_time(e1) / 1e3 / 50) / 1e9
+    del src_t, dst_t
+
     # the same steps as plain per-step launches (no graph)
     S, _, _ = time_steps(sh, torch, S, rule, 10, graph=False)
     S, wall_plain, _ = time_steps(sh, torch, S, rule, args.steps, graph=False)
@@ -126,7 +140,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "life_kernel<16>", "bytes_per_launch": bytes_per_step,
-                     "avg_launch_us": per_step_ev * 1e6},
+                     "avg_launch_us": per_step_ev * 1e6, "stream_copy_gbs": copy_gbs,
+                     "frac_of_copy": achieved / copy_gbs},
         "plain_per_step": {"value": cells * args.steps / wall_plain, "unit": "cells/s",
                            "ms_per_step": wall_plain / args.steps * 1e3,
                            "graph_speedup": wall_plain / wall},

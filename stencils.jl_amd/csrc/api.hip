@@ -80,6 +80,7 @@ int out_dtype(int dt, const sth_op& op, int wdt) {
 struct Plan {
   MapParams p;
   LaunchFn fn;
+  LaunchFn fn_steady;  // valid when the source was written by the previous step
   const char* kname;
   int32_t halo;
   int32_t pad_;
@@ -263,11 +264,31 @@ int make_plan(const sth_array* src, const sth_array* dst, const sth_stencil* st,
   };
   if (!(flags & STH_GENERIC) && op->kind == STH_OP_LIFE && (sdt == STH_U8 || sdt == STH_BOOL) &&
       moore1 && src->ndim == 2 && n[0] % 16 == 0 && aligned16(p.src) && aligned16(p.dst)) {
-    constexpr int KY = 16;
+    // rows per wave: a tuning knob (STH_LIFE_KY overrides the default)
+    int KY = 4;
+    if (const char* e = std::getenv("STH_LIFE_KY")) KY = std::atoi(e);
+    if (KY != 2 && KY != 4 && KY != 8 && KY != 16) KY = 4;
     p.tiles_x = static_cast<int32_t>((n[0] + 1023) / 1024);
     p.tiles_y = static_cast<int32_t>((n[1] + 4 * KY - 1) / (4 * KY));
     p.tiles_z = 1;
-    pl.fn = &launch_life<KY>;
+    const bool conway = op->birth_mask == (1ull << 3) && op->survive_mask == ((1ull << 2) | (1ull << 3));
+    auto pick = [&](bool norm) -> LaunchFn {
+      if (conway && norm)
+        return KY == 2 ? &launch_life<2, true, true> : KY == 4 ? &launch_life<4, true, true>
+             : KY == 16 ? &launch_life<16, true, true> : &launch_life<8, true, true>;
+      if (conway)
+        return KY == 2 ? &launch_life<2, true, false> : KY == 4 ? &launch_life<4, true, false>
+             : KY == 16 ? &launch_life<16, true, false> : &launch_life<8, true, false>;
+      if (norm)
+        return KY == 2 ? &launch_life<2, false, true> : KY == 4 ? &launch_life<4, false, true>
+             : KY == 16 ? &launch_life<16, false, true> : &launch_life<8, false, true>;
+      return KY == 2 ? &launch_life<2, false, false> : KY == 4 ? &launch_life<4, false, false>
+           : KY == 16 ? &launch_life<16, false, false> : &launch_life<8, false, false>;
+    };
+    pl.fn = pick(true);
+    // Every byte this kernel writes is 0 or 1, so later steps of an iteration
+    // can skip normalising the source -- unless they read a halo (Use).
+    pl.fn_steady = (halo_pad && bc == STH_USE) ? pl.fn : pick(false);
     pl.kname = "life_swar";
     if (halo_pad && bc != STH_USE) {
       // map boundaries in registers and refresh the source halo in the same launch
@@ -308,9 +329,10 @@ int make_plan(const sth_array* src, const sth_array* dst, const sth_stencil* st,
     p.tiles_y = static_cast<int32_t>(ty);
     p.tiles_z = static_cast<int32_t>(tz);
     pl.fn = pick->fn;
+    pl.fn_steady = pick->fn;
     pl.kname = pick->name;
   }
-  if (n[0] * n[1] * n[2] == 0) pl.fn = nullptr;  // empty array: nothing to launch
+  if (n[0] * n[1] * n[2] == 0) pl.fn = pl.fn_steady = nullptr;  // empty array: nothing to launch
 
   // ---- halo refresh (update_boundary!) --------------------------------
   if (halo_pad && bc != STH_USE && src->halo > 0 && !p.fuse_halo) {
@@ -341,9 +363,10 @@ int make_plan(const sth_array* src, const sth_array* dst, const sth_stencil* st,
   return STH_OK;
 }
 
-int run_plan(const Plan& pl, hipStream_t s) {
+int run_plan(const Plan& pl, hipStream_t s, bool steady = false) {
   if (pl.halo) pl.halo_fn(pl.hp, s);
-  if (pl.fn) pl.fn(pl.p, s);
+  const LaunchFn f = steady ? pl.fn_steady : pl.fn;
+  if (f) f(pl.p, s);
   STH_HIP(hipGetLastError());
   return STH_OK;
 }
@@ -536,13 +559,18 @@ int sth_iterate(sth_ctx* ctx, const sth_array* a, const sth_array* b, const sth_
   if ((rc = make_plan(b, a, st, boundary, padval, padding, op, flags, &ba))) return rc;
   auto s = static_cast<hipStream_t>(stream);
   *current = static_cast<int32_t>(steps % 2);
+  if (steps == 0) return STH_OK;
+  // The first step runs eagerly (it may read data this library did not
+  // write); steps 1.. use the steady-state kernels.
+  if ((rc = run_plan(ab, s))) return rc;
+  const int64_t rest = steps - 1;
   if (!(flags & STH_GRAPH)) {
-    for (int64_t i = 0; i < steps; ++i)
-      if ((rc = run_plan(i % 2 ? ba : ab, s))) return rc;
+    for (int64_t i = 1; i < steps; ++i)
+      if ((rc = run_plan(i % 2 ? ba : ab, s, true))) return rc;
   } else {
-    // A graph of G = 2U steps (U pairs A->B, B->A) is captured once per
+    // A graph of G = 2U steps starting from buffer b is captured once per
     // parameter set and replayed; the remainder uses a second cached graph.
-    const int64_t U = steps >= 64 ? 16 : std::max<int64_t>(1, steps / 2);
+    const int64_t U = rest >= 64 ? 16 : std::max<int64_t>(1, rest / 2);
     const int64_t G = 2 * U;
     auto get_graph = [&](int64_t nsteps, hipGraphExec_t* exec) -> int {
       std::string key(reinterpret_cast<const char*>(&ab), sizeof(Plan));
@@ -556,7 +584,8 @@ int sth_iterate(sth_ctx* ctx, const sth_array* a, const sth_array* b, const sth_
       }
       STH_HIP(hipStreamBeginCapture(ctx->capture, hipStreamCaptureModeThreadLocal));
       int r2 = STH_OK;
-      for (int64_t i = 0; i < nsteps && r2 == STH_OK; ++i) r2 = run_plan(i % 2 ? ba : ab, ctx->capture);
+      for (int64_t i = 0; i < nsteps && r2 == STH_OK; ++i)
+        r2 = run_plan(i % 2 ? ab : ba, ctx->capture, true);
       hipGraph_t g = nullptr;
       hipError_t e = hipStreamEndCapture(ctx->capture, &g);
       if (r2) return r2;
@@ -569,7 +598,7 @@ int sth_iterate(sth_ctx* ctx, const sth_array* a, const sth_array* b, const sth_
       *exec = x;
       return STH_OK;
     };
-    const int64_t reps = steps / G, rem = steps % G;
+    const int64_t reps = rest / G, rem = rest % G;
     if (reps > 0) {
       hipGraphExec_t x;
       if ((rc = get_graph(G, &x))) return rc;

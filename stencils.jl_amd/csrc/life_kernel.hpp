@@ -50,108 +50,156 @@ __device__ __forceinline__ uint32_t rule_table(uint32_t s, uint32_t c, uint32_t 
   return out;
 }
 
-struct Row16 {
-  uint32_t w[4];  // normalised cells
-  uint32_t h[4];  // horizontal sums (left + centre + right), bytes <= 3
-};
+// One wave's rows.  EDGE selects the general path (rows mapped per boundary,
+// padding rows for Remove); interior waves read consecutive rows.  NORM
+// normalises bytes to 0/1; it can be skipped when the source was written by
+// this kernel (every interior byte is then 0 or 1).
+template <int KY, bool CONWAY, bool NORM, bool EDGE>
+__device__ __forceinline__ void life_rows(const MapParams& p, const uint8_t* __restrict__ src,
+                                          uint8_t* __restrict__ dst, int32_t x0, int32_t y0,
+                                          bool active, int32_t xs, int32_t xlm, int32_t xrm,
+                                          bool lpad, bool rpad) {
+  const int32_t n1 = static_cast<int32_t>(p.src.n1);
+  const int64_t s1 = p.src.s1;
+  const int bc = p.bc;
+  const bool halo = p.read_halo != 0;
+  const uint32_t padn = (p.padval & 0xffu) ? 0x01010101u : 0u;
+  const uint32_t tbl = static_cast<uint32_t>(p.op.birth & 0x1ffu) |
+                       (static_cast<uint32_t>(p.op.survive & 0x1ffu) << 9);
 
-template <int KY>
+  constexpr int NR = KY + 2;
+  uint4 raw[NR];
+  uint32_t lb[NR], rb[NR];
+  bool padrow[NR];
+  if constexpr (!EDGE) {
+    const uint8_t* row = src + static_cast<int64_t>(y0 - 1) * s1;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      padrow[r] = false;
+      raw[r] = *reinterpret_cast<const uint4*>(row + xs);
+      lb[r] = row[xlm];
+      rb[r] = row[xrm];
+      row += s1;
+    }
+  } else {
+    // Rows beyond [-1, n1] exist only in the last, partial block; they are
+    // clamped to a valid row and their results never stored.
+    const bool wrap = bc == STH_WRAP, refl = bc == STH_REFLECT;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const int32_t y = y0 - 1 + r;
+      const int32_t yc = y < -1 ? -1 : (y > n1 ? n1 : y);
+      const bool out = yc < 0 || yc >= n1;
+      const int32_t yw = yc < 0 ? yc + n1 : (yc >= n1 ? yc - n1 : yc);
+      const int32_t yr = yc < 0 ? -yc : (yc >= n1 ? 2 * n1 - 2 - yc : yc);
+      const int32_t ym = halo ? yc : (wrap ? yw : (refl ? yr : (out ? 0 : yc)));
+      padrow[r] = !halo && !wrap && !refl && out;
+      const uint8_t* row = src + static_cast<int64_t>(ym) * s1;
+      raw[r] = *reinterpret_cast<const uint4*>(row + xs);
+      lb[r] = row[xlm];
+      rb[r] = row[xrm];
+    }
+  }
+
+  auto hsum = [&](int r, uint32_t* w, uint32_t* h) {
+    uint32_t v0 = raw[r].x, v1 = raw[r].y, v2 = raw[r].z, v3 = raw[r].w, l = lb[r], rr = rb[r];
+    if constexpr (NORM) {
+      v0 = norm01(v0);
+      v1 = norm01(v1);
+      v2 = norm01(v2);
+      v3 = norm01(v3);
+      l = l ? 1u : 0u;
+      rr = rr ? 1u : 0u;
+    }
+    l = lpad ? (padn & 1u) : l;
+    rr = rpad ? (padn & 1u) : rr;
+    if (EDGE && padrow[r]) {
+      v0 = v1 = v2 = v3 = padn;
+      l = rr = padn & 1u;
+    }
+    w[0] = v0;
+    w[1] = v1;
+    w[2] = v2;
+    w[3] = v3;
+    h[0] = v0 + __builtin_amdgcn_alignbyte(v0, l << 24, 3) + __builtin_amdgcn_alignbyte(v1, v0, 1);
+    h[1] = v1 + __builtin_amdgcn_alignbyte(v1, v0, 3) + __builtin_amdgcn_alignbyte(v2, v1, 1);
+    h[2] = v2 + __builtin_amdgcn_alignbyte(v2, v1, 3) + __builtin_amdgcn_alignbyte(v3, v2, 1);
+    h[3] = v3 + __builtin_amdgcn_alignbyte(v3, v2, 3) + __builtin_amdgcn_alignbyte(rr, v3, 1);
+  };
+
+  uint32_t wa[4], ha[4], wb[4], hb[4], wc[4], hc[4];
+  uint4 outs[KY];
+  hsum(0, wa, ha);
+  hsum(1, wb, hb);
+#pragma unroll
+  for (int j = 0; j < KY; ++j) {
+    hsum(j + 2, wc, hc);
+    uint32_t o[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t s = ha[i] + hb[i] + hc[i] - wb[i];  // Moore count, bytes <= 8
+      o[i] = CONWAY ? rule_conway(s, wb[i]) : rule_table(s, wb[i], tbl);
+    }
+    outs[j] = make_uint4(o[0], o[1], o[2], o[3]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      ha[i] = hb[i];
+      hb[i] = hc[i];
+      wb[i] = wc[i];
+    }
+  }
+  if (active) {
+    uint8_t* d = dst + static_cast<int64_t>(y0) * p.dst.s1 + x0;
+#pragma unroll
+    for (int j = 0; j < KY; ++j) {
+      if (EDGE && y0 + j >= n1) break;
+      // the next step reads this row from memory, not from this CU: stream it
+      __builtin_nontemporal_store(outs[j], reinterpret_cast<uint4*>(d));
+      d += p.dst.s1;
+    }
+  }
+}
+
+template <int KY, bool CONWAY, bool NORM>
 __global__ __launch_bounds__(256) void life_kernel(MapParams p) {
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  // wave index is uniform: keep the row arithmetic on the scalar unit
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nblocks = p.tiles_x * p.tiles_y;
   const int bid = xcd_block_id(static_cast<int>(blockIdx.x), nblocks, p.xcd_remap);
   const int bcol = bid % p.tiles_x;
   const int brow = bid / p.tiles_x;
 
-  const int64_t n0 = p.src.n0, n1 = p.src.n1;
-  const int64_t x0 = (static_cast<int64_t>(bcol) * 64 + lane) * 16;
-  const int64_t y0 = (static_cast<int64_t>(brow) * 4 + wave) * KY;
+  const int32_t n0 = static_cast<int32_t>(p.src.n0), n1 = static_cast<int32_t>(p.src.n1);
+  const int32_t x0 = (bcol * 64 + lane) * 16;
+  const int32_t y0 = (brow * 4 + wave) * KY;
   const bool active = x0 < n0;
   const int bc = p.bc;
   const bool halo = p.read_halo != 0;
-  const uint32_t padn = (p.padval & 0xffu) ? 0x01010101u : 0u;
-  const bool conway = p.op.birth == (1ull << 3) && p.op.survive == ((1ull << 2) | (1ull << 3));
-  const uint32_t tbl = static_cast<uint32_t>(p.op.birth & 0x1ffu) |
-                       (static_cast<uint32_t>(p.op.survive & 0x1ffu) << 9);
   const uint8_t* src = reinterpret_cast<const uint8_t*>(p.src.base);
   uint8_t* dst = reinterpret_cast<uint8_t*>(p.dst.base);
+  const int64_t s1 = p.src.s1;
 
-  // Byte of row y (already mapped / halo row) at column x (-1 .. n0).
-  auto edge_byte = [&](const uint8_t* row, bool padrow, int64_t x) -> uint32_t {
-    if (padrow) return padn & 1u;
-    if (halo || (x >= 0 && x < n0)) return row[x] ? 1u : 0u;
-    if (bc == STH_REMOVE) return padn & 1u;
-    return row[map_index(x, n0, bc)] ? 1u : 0u;
-  };
+  // Column positions of the two edge bytes each lane needs (left of x0,
+  // right of x0 + 15), boundary-mapped; every address stays inside memory so
+  // the loads are unconditional and can all be in flight at once.
+  const int32_t xs = active ? x0 : 0;
+  const int32_t xl = xs - 1, xr = xs + 16;
+  const bool lpad = !halo && xl < 0 && bc == STH_REMOVE;
+  const bool rpad = !halo && xr >= n0 && bc == STH_REMOVE;
+  const int32_t xlm = halo ? xl : (lpad ? 0 : static_cast<int32_t>(map_index(xl, n0, bc)));
+  const int32_t xrm = halo ? xr : (rpad ? 0 : static_cast<int32_t>(map_index(xr, n0, bc)));
 
-  auto load_row = [&](int64_t y, Row16& r) {
-    const bool need = y >= -1 && y <= n1;
-    bool padrow = false;
-    int64_t ym = y;
-    if (!halo && (y < 0 || y >= n1)) {
-      if (bc == STH_REMOVE) padrow = true;
-      else ym = map_index(y, n1, bc);
-    }
-    const uint8_t* row = src + ym * p.src.s1;
-    uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
-    if (need && active) {
-      if (padrow) {
-        w0 = w1 = w2 = w3 = padn;
-      } else {
-        const uint4 v = *reinterpret_cast<const uint4*>(row + x0);
-        w0 = norm01(v.x);
-        w1 = norm01(v.y);
-        w2 = norm01(v.z);
-        w3 = norm01(v.w);
-      }
-    }
-    // Neighbouring lanes hold the adjacent chunks.
-    uint32_t left = __shfl_up(w3, 1) >> 24;
-    uint32_t right = __shfl_down(w0, 1) & 0xffu;
-    if (need && active) {
-      if (lane == 0 || x0 == 0) left = edge_byte(row, padrow, x0 - 1);
-      if (lane == 63 || x0 + 16 == n0) right = edge_byte(row, padrow, x0 + 16);
-    }
-    r.w[0] = w0;
-    r.w[1] = w1;
-    r.w[2] = w2;
-    r.w[3] = w3;
-    r.h[0] = w0 + __builtin_amdgcn_alignbyte(w0, left << 24, 3) + __builtin_amdgcn_alignbyte(w1, w0, 1);
-    r.h[1] = w1 + __builtin_amdgcn_alignbyte(w1, w0, 3) + __builtin_amdgcn_alignbyte(w2, w1, 1);
-    r.h[2] = w2 + __builtin_amdgcn_alignbyte(w2, w1, 3) + __builtin_amdgcn_alignbyte(w3, w2, 1);
-    r.h[3] = w3 + __builtin_amdgcn_alignbyte(w3, w2, 3) + __builtin_amdgcn_alignbyte(right, w3, 1);
-  };
-
-  Row16 a, b, c;
-  load_row(y0 - 1, a);
-  load_row(y0, b);
-#pragma unroll
-  for (int j = 0; j < KY; ++j) {
-    const int64_t y = y0 + j;
-    load_row(y + 1, c);
-    uint4 out;
-    uint32_t o[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const uint32_t s = a.h[i] + b.h[i] + c.h[i] - b.w[i];  // Moore count, bytes <= 8
-      o[i] = conway ? rule_conway(s, b.w[i]) : rule_table(s, b.w[i], tbl);
-    }
-    out.x = o[0];
-    out.y = o[1];
-    out.z = o[2];
-    out.w = o[3];
-    if (active && y < n1) *reinterpret_cast<uint4*>(dst + y * p.dst.s1 + x0) = out;
-    a = b;
-    b = c;
-  }
+  if (y0 >= 1 && y0 + KY < n1)
+    life_rows<KY, CONWAY, NORM, false>(p, src, dst, x0, y0, active, xs, xlm, xrm, lpad, rpad);
+  else
+    life_rows<KY, CONWAY, NORM, true>(p, src, dst, x0, y0, active, xs, xlm, xrm, lpad, rpad);
 
   if (p.fuse_halo && active) {
     // update_boundary! of the source: halo cell (x, y) takes the value of the
     // boundary-mapped interior cell, or padval.
     uint8_t* sw = reinterpret_cast<uint8_t*>(p.src.base);
     const uint8_t pv = static_cast<uint8_t>(p.padval & 0xffu);
-    const int64_t s1 = p.src.s1;
     auto raw_at = [&](int64_t x, int64_t y) -> uint8_t {
       const bool out = x < 0 || x >= n0 || y < 0 || y >= n1;
       if (out && bc == STH_REMOVE) return pv;
@@ -175,10 +223,10 @@ __global__ __launch_bounds__(256) void life_kernel(MapParams p) {
   }
 }
 
-template <int KY>
+template <int KY, bool CONWAY, bool NORM>
 void launch_life(const MapParams& p, hipStream_t s) {
   const unsigned nb = static_cast<unsigned>(p.tiles_x) * p.tiles_y;
-  hipLaunchKernelGGL(life_kernel<KY>, dim3(nb), dim3(256), 0, s, p);
+  hipLaunchKernelGGL((life_kernel<KY, CONWAY, NORM>), dim3(nb), dim3(256), 0, s, p);
 }
 
 }  // namespace sth
