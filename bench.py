@@ -71,6 +71,115 @@ def cpu_baseline(A: np.ndarray, target_s: float = 10.0):
                       f"({dt:.1f} s, oracle/stencil_oracle.c on {cores} threads)"}
 
 
+def graph_time(torch, fn, reps: int, inner: int = 10):
+    """GPU time per call of `fn` (which only enqueues work on the current
+    stream), from a captured torch CUDA graph of `inner` calls replayed
+    `reps` times; returns seconds per call."""
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        fn()  # warm the library's kernel selection outside capture
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        for _ in range(inner):
+            fn()
+    g.replay()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        g.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / 1e3 / (reps * inner)
+
+
+def bench_mean_blur(sh, torch, oracle_ok: bool):
+    """mapstencil(mean, StencilArray(A, Window(1); Remove(0.0), Conditional)),
+    A 1000x1000 Float64 uniform[0,1) (PCG64 seed 1): the README benchmark."""
+    rng = np.random.Generator(np.random.PCG64(1))
+    A = rng.random((1000, 1000))
+    dev = torch.device("cuda", torch.cuda.current_device())
+    t = sh.colmajor_empty(A.shape, torch.float64, dev)
+    t.copy_(torch.from_numpy(np.ascontiguousarray(A)).to(dev))
+    sa = sh.StencilArray(t, sh.Window(1), boundary=sh.Remove(0.0))
+    dest = sh.colmajor_empty(A.shape, torch.float64, dev)
+    per_call = graph_time(torch, lambda: sh.mapstencil_(sh.mean, dest, sa, flags=sh._abi.ASYNC), 20)
+    # end-to-end call from Python (descriptor build, launch, synchronise)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    n = 200
+    for _ in range(n):
+        sh.mapstencil(sh.mean, sa)
+    torch.cuda.synchronize()
+    e2e = (time.perf_counter() - t0) / n
+    cells = A.size
+    out = {"workload": "mean blur 1000x1000 Float64, Window(1), Remove(0.0), Conditional",
+           "kernel_us": per_call * 1e6, "cells_per_s": cells / per_call,
+           "gbs": 16 * cells / per_call / 1e9, "call_us_python": e2e * 1e6,
+           "reference_published_ms": 2.755}
+    if oracle_ok:
+        import oracle
+        B = np.asfortranarray(A)
+        t0 = time.perf_counter()
+        reps = 20
+        for _ in range(reps):
+            oracle.mapstencil(sh.mean, sh.StencilArray(B, sh.Window(1), boundary=sh.Remove(0.0)),
+                              nthreads=oracle.host_cores())
+        out["cpu_oracle_ms"] = (time.perf_counter() - t0) / reps * 1e3
+    return out
+
+
+def gaussian_weights(sigma: float = 1.5, R: int = 3) -> np.ndarray:
+    a = np.arange(-R, R + 1)
+    W = np.exp(-(a[:, None] ** 2 + a[None, :] ** 2) / (2 * sigma ** 2))
+    return W / W.sum()
+
+
+def bench_gaussian(sh, torch, n: int = 16384):
+    """Kernel(Window(3)) 7x7 Gaussian (sigma 1.5) over n x n uniform[0,1)
+    (PCG64 seed 3), Reflect, Halo: Float32, then BF16 storage with f32
+    accumulation; errors against an FP64 evaluation on two 512x512 crops."""
+    import scipy.ndimage as ndi
+    rng = np.random.Generator(np.random.PCG64(3))
+    A = rng.random((n, n), dtype=np.float32)
+    W = gaussian_weights()
+    dev = torch.device("cuda", torch.cuda.current_device())
+    res = {"workload": f"gaussian 7x7 sigma=1.5 {n}x{n}, Reflect, Halo"}
+    crops = [(slice(4000, 4512), slice(9000, 9512)), (slice(0, 509), slice(0, 509))]
+    ref = []
+    A64 = A.astype(np.float64)
+    for cx, cy in crops:
+        lo0, lo1 = max(cx.start - 3, 0), max(cy.start - 3, 0)
+        sub = A64[lo0:cx.stop + 3, lo1:cy.stop + 3]
+        full = ndi.correlate(sub, W, mode="mirror")
+        ref.append(full[cx.start - lo0:cx.start - lo0 + (cx.stop - cx.start),
+                        cy.start - lo1:cy.start - lo1 + (cy.stop - cy.start)])
+    for name, tdt in (("f32", torch.float32), ("bf16", torch.bfloat16)):
+        src = sh.colmajor_empty((n, n), tdt, dev)
+        src.copy_(torch.from_numpy(A).to(dev).to(tdt))
+        K = sh.Kernel(sh.Window(3), W.astype(np.float32))
+        sa = sh.StencilArray(src, K, boundary=sh.Reflect(), padding=sh.Halo())
+        del src
+        dest = sh.colmajor_empty((n, n), tdt, dev)
+        per = graph_time(torch, lambda: sh.mapstencil_(sh.kernelproduct, dest, sa, flags=sh._abi.ASYNC),
+                         3, inner=5)
+        esz = 4 if name == "f32" else 2
+        errs = []
+        for (cx, cy), r in zip(crops, ref):
+            g = dest[cx, cy].float().cpu().numpy().astype(np.float64)
+            errs.append((np.abs(g - r).max(), (np.abs(g - r) / np.abs(r)).max()))
+        res[name] = {"kernel_us": per * 1e6, "cells_per_s": n * n / per,
+                     "gbs": 2 * esz * n * n / per / 1e9,
+                     "max_abs_err_vs_fp64": max(e[0] for e in errs),
+                     "max_rel_err_vs_fp64": max(e[1] for e in errs)}
+        del sa, dest
+        torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +188,8 @@ def main():
     ap.add_argument("--n", type=int, default=8192)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the secondary workloads (mean blur, Gaussian)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
@@ -148,6 +259,9 @@ def main():
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(A, args.cpu_seconds)
+    if not args.no_configs:
+        out["configs"] = {"mean_blur_f64": bench_mean_blur(sh, torch, not args.no_cpu_baseline),
+                          "gaussian_7x7": bench_gaussian(sh, torch)}
     print(json.dumps(out), flush=True)
 
 

@@ -40,7 +40,8 @@
  *
  * BF16 / F16 with STH_OPF_ACC_F32 are this project's storage precision mode
  * (the reference has no BF16 type): values are widened to f32, folded in f32
- * in offset order and rounded once (round-to-nearest-even) at the store.
+ * in offset order (kernelproduct with fused multiply-adds, fmaf) and rounded
+ * once (round-to-nearest-even) at the store.
  * ORC_MODE_F64 computes any float input in double: the FP64 oracle that
  * reduced-precision results are measured against.
  */
@@ -335,8 +336,12 @@ static void cell(const job_t* j, int64_t i0, int64_t i1, int64_t i2) {
         case STH_OP_MAX: acc = k == 0 ? v : jmax_f(acc, v); break;
         case STH_OP_KERNELPRODUCT: {
           float w = load_f32((const char*)j->weights + (size_t)k * 4, STH_F32);
-          float pr = v * w;
-          acc = acc + pr;
+          if (dt == STH_BF16 || dt == STH_F16) {
+            acc = fmaf(v, w, acc); /* storage mode: fused multiply-add */
+          } else {
+            float pr = v * w;
+            acc = acc + pr;
+          }
           break;
         }
       }

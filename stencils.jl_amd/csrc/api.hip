@@ -319,8 +319,9 @@ int make_plan(const sth_array* src, const sth_array* dst, const sth_stencil* st,
     }
     if (!pick) return fail(STH_ENOTSUP, "no kernel for dtype %d op %d", sdt, op->kind);
     const TileDims td = tile_dims(pick->cfg);
+    const int staged = (sdt == STH_BF16 || sdt == STH_F16) ? 4 : dtype_size(sdt);
     const int64_t lds = static_cast<int64_t>(td.tx + 2 * reach[0]) * (td.ty + 2 * reach[1]) *
-                        (td.tz + 2 * reach[2]) * dtype_size(sdt);
+                        (td.tz + 2 * reach[2]) * staged;
     if (lds > 65536) return fail(STH_ENOTSUP, "stencil reach needs %lld bytes of LDS per tile", static_cast<long long>(lds));
     const int64_t tx = (n[0] + td.tx - 1) / td.tx, ty = (n[1] + td.ty - 1) / td.ty,
                   tz = (n[2] + td.tz - 1) / td.tz;

@@ -153,8 +153,8 @@ __device__ __forceinline__ void life_rows(const MapParams& p, const uint8_t* __r
 #pragma unroll
     for (int j = 0; j < KY; ++j) {
       if (EDGE && y0 + j >= n1) break;
-      // the next step reads this row from memory, not from this CU: stream it
-      __builtin_nontemporal_store(outs[j], reinterpret_cast<uint4*>(d));
+      // (plain stores: nontemporal ones measured 45% slower on MI355X here)
+      *reinterpret_cast<uint4*>(d) = outs[j];
       d += p.dst.s1;
     }
   }

@@ -61,6 +61,22 @@ STH_INT_MEAN(uint64_t)
 STH_INT_MEAN(int64_t)
 #undef STH_INT_MEAN
 
+// Element type staged in LDS: BF16 / F16 storage is widened to f32 once,
+// when the tile is loaded, so the fold reads plain floats.
+template <typename S> struct Staged { using type = typename Elem<S>::raw; };
+template <> struct Staged<bf16_t> { using type = float; };
+template <> struct Staged<f16_t> { using type = float; };
+
+template <typename S>
+__device__ __forceinline__ typename Staged<S>::type stage(typename Elem<S>::raw v) {
+  if constexpr (__is_same(S, bf16_t)) return bf16_to_float(v);
+  else if constexpr (__is_same(S, f16_t)) return f16_to_float(v);
+  else return v;
+}
+
+template <typename S>
+constexpr bool kStorageMode = __is_same(S, bf16_t) || __is_same(S, f16_t);
+
 // A stencil whose offsets arrive at run time (Positional, NamedStencil and
 // shapes outside the build-time catalogue).
 struct RuntimeShape {
@@ -93,10 +109,10 @@ __device__ __forceinline__ A fold_max(A a, A b) {
 // One output value from the staged box; `c` is the LDS index of the centre.
 template <typename S, int OPK, class Sh, int W, int HW>
 __device__ __forceinline__ typename AccOf<S, OPK>::type reduce_cell(
-    const typename Elem<S>::raw* __restrict__ tile, int c, const MapParams& p,
+    const typename Staged<S>::type* __restrict__ tile, int c, const MapParams& p,
     int wdyn, int hwdyn) {
   using A = typename AccOf<S, OPK>::type;
-  using Raw = typename Elem<S>::raw;
+  using Raw = typename Staged<S>::type;
   const int L = Sh::kStatic ? Sh::L : p.L;
   A acc = A(0);
   uint32_t cnt = 0;
@@ -105,7 +121,9 @@ __device__ __forceinline__ typename AccOf<S, OPK>::type reduce_cell(
     if constexpr (OPK == OP_LIFE) {
       cnt += nonzero(rv) ? 1u : 0u;
     } else {
-      const A v = widen<A, S>(rv);
+      A v;
+      if constexpr (kStorageMode<S>) v = rv;
+      else v = widen<A, S>(rv);
       if constexpr (OPK == OP_SUM || OPK == OP_MEAN || OPK == OP_LAP) {
         acc = k == 0 ? v : acc + v;
       } else if constexpr (OPK == OP_MIN) {
@@ -113,8 +131,14 @@ __device__ __forceinline__ typename AccOf<S, OPK>::type reduce_cell(
       } else if constexpr (OPK == OP_MAX) {
         acc = k == 0 ? v : fold_max(acc, v);
       } else if constexpr (OPK == OP_KP) {
-        const A pr = v * weight<A>(p, k);
-        acc = acc + pr;
+        if constexpr (kStorageMode<S>) {
+          // storage precision mode: f32 fused multiply-add, offset order
+          acc = __builtin_fmaf(v, weight<A>(p, k), acc);
+        } else {
+          // native types: separate multiply and add, as the reference
+          const A pr = v * weight<A>(p, k);
+          acc = acc + pr;
+        }
       }
     }
   };
@@ -134,7 +158,9 @@ __device__ __forceinline__ typename AccOf<S, OPK>::type reduce_cell(
   if constexpr (OPK == OP_MEAN) {
     acc = acc / static_cast<A>(L);
   } else if constexpr (OPK == OP_LAP) {
-    const A cv = widen<A, S>(tile[c]);
+    A cv;
+    if constexpr (kStorageMode<S>) cv = tile[c];
+    else cv = widen<A, S>(tile[c]);
     const A t = static_cast<A>(L) * cv;
     const A d = acc - t;
     const A a = static_cast<A>(p.op.alpha) * d;
@@ -156,6 +182,7 @@ __device__ __forceinline__ Raw load_raw(const DevArr& a, int64_t x, int64_t y, i
 template <typename S, int OPK, class Sh, int TX, int TY, int TZ>
 __global__ __launch_bounds__(256) void tile_kernel(MapParams p) {
   using Raw = typename Elem<S>::raw;
+  using T = typename Staged<S>::type;
   using O = typename OutOf<S, OPK>::type;
   using ORaw = typename Elem<O>::raw;
   constexpr int kThreads = 256;
@@ -174,7 +201,7 @@ __global__ __launch_bounds__(256) void tile_kernel(MapParams p) {
   const int HW = H * W;
 
   extern __shared__ __align__(16) unsigned char smem_dyn[];
-  Raw* tile = reinterpret_cast<Raw*>(smem_dyn);
+  T* tile = reinterpret_cast<T*>(smem_dyn);
 
   const int nblocks = p.tiles_x * p.tiles_y * p.tiles_z;
   const int bid = xcd_block_id(static_cast<int>(blockIdx.x), nblocks, p.xcd_remap);
@@ -203,7 +230,7 @@ __global__ __launch_bounds__(256) void tile_kernel(MapParams p) {
       const int r = idx - lz * HW;
       const int ly = r / W;
       const int lx = r - ly * W;
-      tile[idx] = load_raw<Raw>(p.src, X0 + lx, Y0 + ly, Z0 + lz);
+      tile[idx] = stage<S>(load_raw<Raw>(p.src, X0 + lx, Y0 + ly, Z0 + lz));
     }
   } else {
     const Raw pv = raw_from_bits<Raw>(p.padval);
@@ -229,7 +256,7 @@ __global__ __launch_bounds__(256) void tile_kernel(MapParams p) {
                             map_index(gz, n2, p.bc));
         }
       }
-      tile[idx] = v;
+      tile[idx] = stage<S>(v);
     }
   }
   __syncthreads();
@@ -256,7 +283,7 @@ __global__ __launch_bounds__(256) void tile_kernel(MapParams p) {
 // Host-side launcher with a uniform signature for the kernel registry.
 template <typename S, int OPK, class Sh, int TX, int TY, int TZ>
 void launch_tile(const MapParams& p, hipStream_t stream) {
-  using Raw = typename Elem<S>::raw;
+  using Raw = typename Staged<S>::type;
   const int rx = Sh::kStatic ? Sh::RX : p.rx;
   const int ry = Sh::kStatic ? Sh::RY : p.ry;
   const int rz = Sh::kStatic ? Sh::RZ : p.rz;

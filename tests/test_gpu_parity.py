@@ -184,7 +184,9 @@ def test_parity_storage_types(gpu, oracle, tname):
             pv = b"\0\0" if bname == "remove" else None
             o = oracle.map_raw(np.asfortranarray(raw), code, A32.shape, 0, st.offsets(), 2, bc.code,
                                pv, 0, sh.kernelproduct.op(sh._abi.OPF_ACC_F32), K.weights, sh._abi.F32)
-            np.testing.assert_array_equal(g, o.view(np.uint16))
+            bad = np.argwhere(g != o.view(np.uint16))
+            np.testing.assert_array_equal(g, o.view(np.uint16),
+                                          err_msg=f"{st} {bname} kernelproduct at {bad[:4].tolist()}")
 
 
 # ---------------------------------------------------- iteration / Life
