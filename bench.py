@@ -180,6 +180,70 @@ def bench_gaussian(sh, torch, n: int = 16384):
     return res
 
 
+def batched_workload(sh, torch, n: int = 512, seed: int = 4):
+    """512 Float32 arrays, side s log-uniform in [64, 4096], aspect a in
+    [0.5, 2] (shape (s, round(s a))), each with a uniform choice of
+    {Window(1), Circle(3), VonNeumann(2)} x {sum, maximum, mean},
+    Remove(0f0), Conditional; data uniform[0,1) from a seeded device RNG."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    shapes = [sh.Window(1), sh.Circle(3), sh.VonNeumann(2)]
+    fs = [sh.sum, sh.maximum, sh.mean]
+    calls = []
+    for _ in range(n):
+        s = int(round(np.exp(rng.uniform(np.log(64), np.log(4096)))))
+        a = rng.uniform(0.5, 2.0)
+        shape = (s, max(8, int(round(s * a))))
+        st = shapes[int(rng.integers(3))]
+        f = fs[int(rng.integers(3))]
+        t = sh.colmajor_empty(shape, torch.float32, dev)
+        t.copy_(torch.rand(shape[::-1], generator=gen, device=dev).T)
+        sa = sh.StencilArray(t, st, boundary=sh.Remove(np.float32(0)))
+        calls.append((f, sh.colmajor_empty(shape, torch.float32, dev), sa))
+    return calls
+
+
+def bench_batched(sh, torch):
+    """Config 4: one persistent launch (sth_map_batch) vs one launch per array."""
+    calls = batched_workload(sh, torch)
+    plan = sh.BatchPlan(calls)
+    cells = plan.cells
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps, e0.elapsed_time(e1) / 1e3 / reps
+
+    wall_b, gpu_b = timed(lambda: plan.run(flags=sh._abi.ASYNC), 5)
+    wall_s, gpu_s = timed(lambda: plan.run(serial=True, flags=sh._abi.ASYNC), 5)
+
+    def python_loop():
+        for f, d, sa in calls:
+            sh.mapstencil_(f, d, sa, flags=sh._abi.ASYNC)
+    wall_p, gpu_p = timed(python_loop, 2)
+    # spot check: the batched results equal single calls
+    plan.run()
+    for f, d, sa in calls[:8]:
+        assert torch.equal(d, sh.mapstencil(f, sa)), "batched result differs from a single call"
+    return {"workload": "512 uneven Float32 arrays (sides log-uniform 64..4096, aspect 0.5..2), "
+                        "Window(1)/Circle(3)/VonNeumann(2) x sum/maximum/mean, Remove(0f0), Conditional",
+            "cells": cells, "bytes": 8 * cells,
+            "batched": {"ms": wall_b * 1e3, "gpu_ms": gpu_b * 1e3, "cells_per_s": cells / wall_b,
+                        "gbs": 8 * cells / gpu_b / 1e9},
+            "per_array_launches_c": {"ms": wall_s * 1e3, "gpu_ms": gpu_s * 1e3,
+                                     "cells_per_s": cells / wall_s},
+            "per_array_python_loop": {"ms": wall_p * 1e3, "cells_per_s": cells / wall_p},
+            "speedup_vs_c_loop": wall_s / wall_b, "speedup_vs_python_loop": wall_p / wall_b}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -261,7 +325,8 @@ def main():
         out["cpu_baseline"] = cpu_baseline(A, args.cpu_seconds)
     if not args.no_configs:
         out["configs"] = {"mean_blur_f64": bench_mean_blur(sh, torch, not args.no_cpu_baseline),
-                          "gaussian_7x7": bench_gaussian(sh, torch)}
+                          "gaussian_7x7": bench_gaussian(sh, torch),
+                          "batched_uneven": bench_batched(sh, torch)}
     print(json.dumps(out), flush=True)
 
 

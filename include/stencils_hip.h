@@ -192,6 +192,32 @@ int sth_iterate(sth_ctx* ctx, const sth_array* a, const sth_array* b,
                 int32_t padding, const sth_op* op, int64_t steps,
                 uint32_t flags, int32_t* current, void* stream);
 
+/* One mapstencil call of a batch (sth_map_batch). */
+typedef struct sth_job {
+  sth_array src;
+  sth_array dst;
+  sth_stencil stencil;
+  sth_op op;
+  const void* padval;
+  int32_t boundary;
+  int32_t padding;
+} sth_job;
+
+#define STH_BATCH_SERIAL 8u /* sth_map_batch: one launch per job (the unbatched path) */
+
+/*
+ * Many independent gatherstencil!(f, dest, source) calls -- mixed sizes,
+ * shapes and reductions -- in one persistent launch per element type.  Jobs
+ * outside the batched catalogue (and Halo jobs that need a refresh) run as
+ * individual launches inside the call.  Each result is identical to an
+ * sth_map call on that job alone.  No counterpart in the reference, which
+ * runs one KernelAbstractions launch per mapstencil call
+ * (src/gatherstencil.jl:97-100); callers: the Julia shim's batched
+ * `mapstencil` over a vector of StencilArrays, and the serving loop.
+ */
+int sth_map_batch(sth_ctx* ctx, int32_t njobs, const sth_job* jobs, uint32_t flags,
+                  void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include "batch_kernel.hpp"
 #include "halo_kernel.hpp"
 #include "life_kernel.hpp"
 #include "registry.hpp"
@@ -23,6 +24,14 @@ namespace sth {
 
 std::vector<KernelEntry>& registry() {
   static std::vector<KernelEntry> r;
+  return r;
+}
+std::vector<BatchVariant>& batch_variants() {
+  static std::vector<BatchVariant> r;
+  return r;
+}
+std::vector<BatchKernelEntry>& batch_kernels() {
+  static std::vector<BatchKernelEntry> r;
   return r;
 }
 
@@ -375,11 +384,24 @@ int run_plan(const Plan& pl, hipStream_t s, bool steady = false) {
 }  // namespace
 }  // namespace sth
 
+// Job-table slot of sth_map_batch: pinned host staging + device copy; the
+// event marks the launch that last read it.
+struct BatchSlot {
+  void* host = nullptr;
+  void* dev = nullptr;
+  size_t bytes = 0;
+  hipEvent_t done = nullptr;
+  bool used = false;
+};
+
 struct sth_ctx {
   int device = 0;
+  int cus = 0;
   hipStream_t capture = nullptr;  // private stream used only for graph capture
   std::mutex mu;
   std::map<std::string, hipGraphExec_t> graphs;
+  BatchSlot slots[4];
+  unsigned next_slot = 0;
 };
 
 using namespace sth;
@@ -417,6 +439,8 @@ int sth_ctx_create(int device, sth_ctx** out) {
   STH_HIP(hipSetDevice(device));
   auto* c = new sth_ctx();
   c->device = device;
+  if (hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || c->cus < 1)
+    c->cus = 256;
   hipError_t e = hipStreamCreateWithFlags(&c->capture, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete c;
@@ -433,6 +457,12 @@ int sth_ctx_destroy(sth_ctx* ctx) {
   if (!ctx) return STH_OK;
   (void)hipSetDevice(ctx->device);
   for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto& sl : ctx->slots) {
+    if (sl.done) (void)hipEventSynchronize(sl.done);
+    if (sl.done) (void)hipEventDestroy(sl.done);
+    if (sl.host) (void)hipHostFree(sl.host);
+    if (sl.dev) (void)hipFree(sl.dev);
+  }
   if (ctx->capture) (void)hipStreamDestroy(ctx->capture);
   delete ctx;
   return STH_OK;
@@ -612,6 +642,111 @@ int sth_iterate(sth_ctx* ctx, const sth_array* a, const sth_array* b, const sth_
     }
   }
   STH_HIP(hipGetLastError());
+  if (!(flags & STH_ASYNC)) STH_HIP(hipStreamSynchronize(s));
+  return STH_OK;
+  STH_GUARD_END
+}
+
+int sth_map_batch(sth_ctx* ctx, int32_t njobs, const sth_job* jobs, uint32_t flags, void* stream) {
+  STH_GUARD_BEGIN
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  if (njobs < 0 || (njobs > 0 && !jobs)) return fail(STH_EINVAL, "bad job list");
+  auto s = static_cast<hipStream_t>(stream);
+  std::vector<Plan> plans(static_cast<size_t>(njobs));
+  for (int32_t i = 0; i < njobs; ++i) {
+    const sth_job& jb = jobs[i];
+    if ((rc = make_plan(&jb.src, &jb.dst, &jb.stencil, jb.boundary, jb.padval, jb.padding, &jb.op,
+                        flags & ~STH_GENERIC, &plans[i]))) {
+      std::string m = g_err;
+      return fail(rc, "job %d: %s", i, m.c_str());
+    }
+  }
+  // group the batchable jobs by element type
+  std::map<int, std::vector<BatchJob>> groups;
+  std::vector<int32_t> serial;
+  for (int32_t i = 0; i < njobs; ++i) {
+    const sth_job& jb = jobs[i];
+    const Plan& pl = plans[i];
+    const int dt = jb.src.dtype;
+    const BatchVariant* v = nullptr;
+    if (!(flags & STH_BATCH_SERIAL) && !pl.halo && pl.fn &&
+        (dt == STH_F32 || dt == STH_F64) && jb.op.kind <= STH_OP_MAX && jb.stencil.length <= 127) {
+      const int L = jb.stencil.length, nd = jb.stencil.ndim;
+      for (const auto& bv : batch_variants()) {
+        if (bv.dtype != dt || bv.op != jb.op.kind || bv.L != L || bv.ndim != nd) continue;
+        bool eq = true;
+        for (int k = 0; k < L && eq; ++k)
+          for (int d = 0; d < 3 && eq; ++d) {
+            const int o = d < nd ? jb.stencil.offsets[k * nd + d] : 0;
+            eq = bv.offs[3 * k + d] == o;
+          }
+        if (eq) {
+          v = &bv;
+          break;
+        }
+      }
+    }
+    if (!v) {
+      if (pl.fn || pl.halo) serial.push_back(i);
+      continue;
+    }
+    BatchJob bj;
+    std::memset(static_cast<void*>(&bj), 0, sizeof bj);
+    bj.src = pl.p.src;
+    bj.dst = pl.p.dst;
+    bj.bc = pl.p.bc;
+    bj.read_halo = pl.p.read_halo;
+    bj.padval = pl.p.padval;
+    bj.op = pl.p.op;
+    bj.variant = v->variant;
+    bj.tiles_x = static_cast<int32_t>((pl.p.src.n0 + v->tx - 1) / v->tx);
+    bj.tiles_y = static_cast<int32_t>((pl.p.src.n1 + v->ty - 1) / v->ty);
+    bj.tiles_z = static_cast<int32_t>(pl.p.src.n2);
+    groups[dt].push_back(bj);
+  }
+  for (int32_t i : serial)
+    if ((rc = run_plan(plans[i], s))) return rc;
+  for (auto& kv : groups) {
+    const BatchKernelEntry* ke = nullptr;
+    for (const auto& e : batch_kernels())
+      if (e.dtype == kv.first) ke = &e;
+    if (!ke) return fail(STH_EDEVICE, "no batched kernel for dtype %d", kv.first);
+    std::vector<BatchJob>& bj = kv.second;
+    int64_t total = 0;
+    for (auto& j : bj) {
+      j.tile_begin = total;
+      total += static_cast<int64_t>(j.tiles_x) * j.tiles_y * j.tiles_z;
+    }
+    if (total == 0) continue;
+    const size_t bytes = bj.size() * sizeof(BatchJob);
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    BatchSlot& sl = ctx->slots[ctx->next_slot++ % 4];
+    if (sl.used) STH_HIP(hipEventSynchronize(sl.done));  // its last launch has read it
+    if (sl.bytes < bytes) {
+      if (sl.host) STH_HIP(hipHostFree(sl.host));
+      if (sl.dev) STH_HIP(hipFree(sl.dev));
+      sl.host = sl.dev = nullptr;
+      sl.bytes = 0;
+      STH_HIP(hipHostMalloc(&sl.host, bytes, hipHostMallocDefault));
+      STH_HIP(hipMalloc(&sl.dev, bytes));
+      sl.bytes = bytes;
+    }
+    if (!sl.done) STH_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+    std::memcpy(sl.host, bj.data(), bytes);
+    STH_HIP(hipMemcpyAsync(sl.dev, sl.host, bytes, hipMemcpyHostToDevice, s));
+    BatchParams bp;
+    bp.jobs = static_cast<const BatchJob*>(sl.dev);
+    bp.njobs = static_cast<int32_t>(bj.size());
+    bp.pad_ = 0;
+    bp.total_tiles = total;
+    const int64_t resident = static_cast<int64_t>(ke->occupancy(ke->lds)) * ctx->cus;
+    const unsigned blocks = static_cast<unsigned>(std::min<int64_t>(total, resident));
+    ke->fn(bp, blocks, ke->lds, s);
+    STH_HIP(hipGetLastError());
+    STH_HIP(hipEventRecord(sl.done, s));
+    sl.used = true;
+  }
   if (!(flags & STH_ASYNC)) STH_HIP(hipStreamSynchronize(s));
   return STH_OK;
   STH_GUARD_END

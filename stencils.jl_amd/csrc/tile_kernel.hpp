@@ -84,8 +84,8 @@ struct RuntimeShape {
   static constexpr int L = 0, RX = 0, RY = 0, RZ = 0;
 };
 
-template <typename A>
-__device__ __forceinline__ A weight(const MapParams& p, int k) {
+template <typename A, class P>
+__device__ __forceinline__ A weight(const P& p, int k) {
   if constexpr (__is_same(A, double)) {
     return p.w.d[k];
   } else if constexpr (__is_same(A, float)) {
@@ -107,9 +107,9 @@ __device__ __forceinline__ A fold_max(A a, A b) {
 }
 
 // One output value from the staged box; `c` is the LDS index of the centre.
-template <typename S, int OPK, class Sh, int W, int HW>
+template <typename S, int OPK, class Sh, int W, int HW, class P>
 __device__ __forceinline__ typename AccOf<S, OPK>::type reduce_cell(
-    const typename Staged<S>::type* __restrict__ tile, int c, const MapParams& p,
+    const typename Staged<S>::type* __restrict__ tile, int c, const P& p,
     int wdyn, int hwdyn) {
   using A = typename AccOf<S, OPK>::type;
   using Raw = typename Staged<S>::type;
@@ -179,10 +179,13 @@ __device__ __forceinline__ Raw load_raw(const DevArr& a, int64_t x, int64_t y, i
   return p[z * a.s2 + y * a.s1 + x];
 }
 
-template <typename S, int OPK, class Sh, int TX, int TY, int TZ>
-__global__ __launch_bounds__(256) void tile_kernel(MapParams p) {
+// One output tile: stage the source box in LDS, fold, store.  `P` carries
+// the array descriptors, boundary and op parameters (MapParams for a single
+// array, BatchJob for one job of a batch); (bx, by, bz) is the tile.
+template <typename S, int OPK, class Sh, int TX, int TY, int TZ, class P>
+__device__ __forceinline__ void tile_body(const P& p, int bx, int by, int bz,
+                                          typename Staged<S>::type* __restrict__ tile) {
   using Raw = typename Elem<S>::raw;
-  using T = typename Staged<S>::type;
   using O = typename OutOf<S, OPK>::type;
   using ORaw = typename Elem<O>::raw;
   constexpr int kThreads = 256;
@@ -199,16 +202,6 @@ __global__ __launch_bounds__(256) void tile_kernel(MapParams p) {
   const int H = Sh::kStatic ? HS : TY + 2 * ry;
   const int D = Sh::kStatic ? DS : TZ + 2 * rz;
   const int HW = H * W;
-
-  extern __shared__ __align__(16) unsigned char smem_dyn[];
-  T* tile = reinterpret_cast<T*>(smem_dyn);
-
-  const int nblocks = p.tiles_x * p.tiles_y * p.tiles_z;
-  const int bid = xcd_block_id(static_cast<int>(blockIdx.x), nblocks, p.xcd_remap);
-  const int bx = bid % p.tiles_x;
-  const int byz = bid / p.tiles_x;
-  const int by = byz % p.tiles_y;
-  const int bz = byz / p.tiles_y;
 
   const int64_t n0 = p.src.n0, n1 = p.src.n1, n2 = p.src.n2;
   const int64_t X0 = static_cast<int64_t>(bx) * TX - rx;
@@ -278,6 +271,19 @@ __global__ __launch_bounds__(256) void tile_kernel(MapParams p) {
     if (gx < n0 && gy < n1 && gz < n2)
       dstp[gz * p.dst.s2 + gy * p.dst.s1 + gx] = narrow<O>(acc);
   }
+}
+
+template <typename S, int OPK, class Sh, int TX, int TY, int TZ>
+__global__ __launch_bounds__(256) void tile_kernel(MapParams p) {
+  extern __shared__ __align__(16) unsigned char smem_dyn[];
+  auto* tile = reinterpret_cast<typename Staged<S>::type*>(smem_dyn);
+  const int nblocks = p.tiles_x * p.tiles_y * p.tiles_z;
+  const int bid = xcd_block_id(static_cast<int>(blockIdx.x), nblocks, p.xcd_remap);
+  const int bx = bid % p.tiles_x;
+  const int byz = bid / p.tiles_x;
+  const int by = byz % p.tiles_y;
+  const int bz = byz / p.tiles_y;
+  tile_body<S, OPK, Sh, TX, TY, TZ>(p, bx, by, bz, tile);
 }
 
 // Host-side launcher with a uniform signature for the kernel registry.

@@ -9,8 +9,8 @@ from ._lib import BackendUnavailable, StencilsHipError, load_library
 from .array import (StencilArray, SwitchingStencilArray, colmajor_empty, dest,
                     parent, source, switch)
 from .boundary import Conditional, Halo, Reflect, Remove, Use, Wrap, padval
-from .gather import (gatherstencil, gatherstencil_, mapstencil, mapstencil_,
-                     mapstencil_steps, result_dtype)
+from .gather import (BatchPlan, gatherstencil, gatherstencil_, mapstencil, mapstencil_,
+                     mapstencil_batch, mapstencil_steps, result_dtype)
 from .ops import (Laplacian, LifeRule, Unsupported, kernelproduct, maximum, mean,
                   minimum, sum)
 from .stencil import (AngledCross, Annulus, BackSlash, Cardinal, Circle, Cross,

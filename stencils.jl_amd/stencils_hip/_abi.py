@@ -42,6 +42,7 @@ OPF_ACC_F32 = 1
 ASYNC = 1
 GRAPH = 2
 GENERIC = 4
+BATCH_SERIAL = 8
 
 
 class SthArray(ctypes.Structure):
@@ -76,6 +77,18 @@ class SthOp(ctypes.Structure):
         ("alpha", ctypes.c_double),
         ("birth_mask", ctypes.c_uint64),
         ("survive_mask", ctypes.c_uint64),
+    ]
+
+
+class SthJob(ctypes.Structure):
+    _fields_ = [
+        ("src", SthArray),
+        ("dst", SthArray),
+        ("stencil", SthStencil),
+        ("op", SthOp),
+        ("padval", ctypes.c_void_p),
+        ("boundary", ctypes.c_int32),
+        ("padding", ctypes.c_int32),
     ]
 
 

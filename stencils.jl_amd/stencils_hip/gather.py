@@ -290,6 +290,63 @@ def mapstencil_steps(f, A: SwitchingStencilArray, steps: int, graph: bool = True
     return A.switch() if cur.value == 1 else A
 
 
+class BatchPlan:
+    """Descriptors of many gatherstencil!(f, dest, source) calls, built once
+    (device tensors only) and run with sth_map_batch as often as needed."""
+
+    def __init__(self, calls):
+        _require_backend()
+        calls = list(calls)
+        self.n = len(calls)
+        self.jobs = (_abi.SthJob * max(self.n, 1))()
+        self._keep = _Call()
+        self.dev = None
+        self.cells = 0
+        for i, (f, dest, sa) in enumerate(calls):
+            red = resolve(f)
+            if not (is_device(dest) and is_device(sa.parent())):
+                raise Unsupported("batched jobs must live on the device")
+            dt = sa.sth_dtype()
+            j = self.jobs[i]
+            j.src = _desc(sa.parent(), sa.size(), sa.halo())
+            j.dst = _desc(dest, sa.size(), 0)
+            j.stencil = stencil_descriptor(sa.stencil, dt, self._keep)
+            j.op = red.op(_op_flags(dt))
+            pv = _padval_bytes(sa)
+            if pv is not None:
+                buf = self._keep.hold(ctypes.create_string_buffer(pv, len(pv)))
+                j.padval = ctypes.cast(buf, ctypes.c_void_p)
+            j.boundary = sa.boundary.code
+            j.padding = sa.padding.code
+            self._keep.hold((dest, sa))
+            c = 1
+            for s in sa.size():
+                c *= s
+            self.cells += c
+            d = _device_index(sa.parent())
+            if self.dev is None:
+                self.dev = d
+            elif d != self.dev:
+                raise ValueError("all jobs of a batch must be on one device")
+
+    def run(self, serial: bool = False, flags: int = 0):
+        if self.n == 0:
+            return
+        L = _lib.lib()
+        fl = flags | (_abi.BATCH_SERIAL if serial else 0)
+        _lib.check(L.sth_map_batch(_lib.context(self.dev), self.n, self.jobs, fl,
+                                   _stream(self.dev)))
+
+
+def mapstencil_batch(calls, serial: bool = False, flags: int = 0):
+    """Many gatherstencil!(f, dest, source) calls in one library call
+    (sth_map_batch): jobs of mixed sizes, shapes and reductions share one
+    persistent launch per element type.  `calls` is a sequence of
+    (f, dest, source) with device tensors; `serial` runs one launch per job
+    instead (the unbatched path).  Results equal the one-call-per-job path."""
+    BatchPlan(calls).run(serial=serial, flags=flags)
+
+
 # Deprecated aliases of the reference (src/gatherstencil.jl:127-128).
 mapstencil = gatherstencil
 mapstencil_ = gatherstencil_

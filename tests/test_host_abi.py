@@ -46,6 +46,8 @@ int main(void) {
          offsetof(sth_stencil, weights), offsetof(sth_stencil, weights_dtype));
   printf("%zu %zu %zu %zu\n", sizeof(sth_op), offsetof(sth_op, alpha),
          offsetof(sth_op, birth_mask), offsetof(sth_op, survive_mask));
+  printf("%zu %zu %zu %zu %zu\n", sizeof(sth_job), offsetof(sth_job, stencil),
+         offsetof(sth_job, op), offsetof(sth_job, padval), offsetof(sth_job, padding));
   return 0;
 }''')
     exe = tmp_path / "layout"
@@ -55,6 +57,8 @@ int main(void) {
     assert out[0] == [ctypes.sizeof(A), A.size.offset, A.stride.offset, A.halo.offset, A.dtype.offset]
     assert out[1] == [ctypes.sizeof(S), S.offsets.offset, S.weights.offset, S.weights_dtype.offset]
     assert out[2] == [ctypes.sizeof(O), O.alpha.offset, O.birth_mask.offset, O.survive_mask.offset]
+    J = _abi.SthJob
+    assert out[3] == [ctypes.sizeof(J), J.stencil.offset, J.op.offset, J.padval.offset, J.padding.offset]
 
 
 def test_header_enums_match_python():
