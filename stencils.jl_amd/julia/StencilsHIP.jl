@@ -1,0 +1,262 @@
+"""
+    StencilsHIP
+
+`ccall` shim that routes Stencils.jl's `gatherstencil!` (and so `mapstencil!`)
+to `libstencils_hip.so` on an MI355X. It applies to `StencilArray`s whose
+parent is a `ROCArray` and whose function is one of the fixed reductions:
+`sum`, `mean`, `minimum`, `maximum`, `kernelproduct`, [`LifeRule`] and
+[`Laplacian`]. Every other call, and every combination the library reports
+as `STH_ENOTSUP`, is `invoke`d on the reference method
+(src/gatherstencil.jl:89-103). Argument errors come back as `ArgumentError`,
+as in the reference.
+
+Load it after Stencils and AMDGPU:
+
+    using Stencils, AMDGPU, Statistics
+    include("StencilsHIP.jl"); using .StencilsHIP
+
+The C structs below must match `include/stencils_hip.h` field for field.
+"""
+module StencilsHIP
+
+using Stencils
+using Statistics: mean
+using Stencils: AbstractStencilArray, StencilArray, SwitchingStencilArray, Stencil,
+    AbstractKernelStencil, Remove, Wrap, Reflect, Use, Halo, Conditional,
+    kernelproduct, offsets, radius, padding, boundary, stencil, source, dest, switch,
+    neighbors, center, kernel
+using AMDGPU: ROCArray
+
+export LifeRule, Laplacian, mapstencil_steps!, sth_available
+
+const libsth = get(ENV, "STENCILS_HIP_LIB",
+                   joinpath(@__DIR__, "..", "lib", "libstencils_hip.so"))
+
+# ---------------------------------------------------------------- status / enums
+const STH_OK, STH_EDEVICE, STH_ENOMEM, STH_EINVAL, STH_ENOTSUP = 0, 5, 12, 22, 95
+const STH_ASYNC, STH_GRAPH = UInt32(1), UInt32(2)
+const STH_OPF_ACC_F32 = UInt32(1)
+
+const DTYPES = Dict{DataType,Int32}(
+    Bool => 0, UInt8 => 1, Int8 => 2, UInt16 => 3, Int16 => 4, UInt32 => 5, Int32 => 6,
+    UInt64 => 7, Int64 => 8, Float16 => 9, Float32 => 11, Float64 => 12)
+
+_bc(::Remove) = Int32(0)
+_bc(::Wrap) = Int32(1)
+_bc(::Reflect) = Int32(2)
+_bc(::Use) = Int32(3)
+_pad(::Conditional) = Int32(0)
+_pad(::Halo{:out}) = Int32(1)
+_pad(::Halo{:in}) = Int32(2)
+
+# ------------------------------------------------------------------ C structs
+struct SthArray
+    data::Ptr{Cvoid}
+    dtype::Int32
+    ndim::Int32
+    size::NTuple{3,Int64}
+    stride::NTuple{3,Int64}
+    halo::Int32
+    reserved::Int32
+end
+
+struct SthStencil
+    kind::Int32
+    radius::Int32
+    ndim::Int32
+    length::Int32
+    offsets::Ptr{Int32}
+    weights::Ptr{Cvoid}
+    weights_dtype::Int32
+    reserved::Int32
+end
+
+struct SthOp
+    kind::Int32
+    flags::UInt32
+    alpha::Float64
+    birth_mask::UInt64
+    survive_mask::UInt64
+end
+
+# ---------------------------------------------------- canonical rule callables
+"""
+    LifeRule(; birth=(3,), survive=(2, 3))
+
+Life-like rule over Moore(1): a cell with `s` nonzero neighbours becomes
+`one(T)` if it is dead and `s ∈ birth`, or alive and `s ∈ survive`.
+"""
+struct LifeRule
+    birth::UInt64
+    survive::UInt64
+end
+LifeRule(; birth=(3,), survive=(2, 3)) =
+    LifeRule(reduce(|, (UInt64(1) << b for b in birth); init=UInt64(0)),
+             reduce(|, (UInt64(1) << s for s in survive); init=UInt64(0)))
+function (r::LifeRule)(hood)
+    s = count(!iszero, neighbors(hood))
+    mask = iszero(center(hood)) ? r.birth : r.survive
+    T = typeof(center(hood))
+    return T((mask >> s) & 1)
+end
+
+"""
+    Laplacian(α)
+
+Explicit diffusion step `c + α * (sum(neighbours) - L * c)` over a
+VonNeumann(1, N) stencil, each operation rounded in the element type.
+"""
+struct Laplacian
+    alpha::Float64
+end
+function (l::Laplacian)(hood)
+    c = center(hood)
+    T = typeof(c)
+    return c + T(l.alpha) * (sum(neighbors(hood)) - length(hood) * c)
+end
+
+# --------------------------------------------------------- op descriptors
+_op(::typeof(sum), T) = SthOp(0, _accflag(T), 0.0, 0, 0)
+_op(::typeof(mean), T) = SthOp(1, _accflag(T), 0.0, 0, 0)
+_op(::typeof(minimum), T) = SthOp(2, _accflag(T), 0.0, 0, 0)
+_op(::typeof(maximum), T) = SthOp(3, _accflag(T), 0.0, 0, 0)
+_op(::typeof(kernelproduct), T) = SthOp(4, _accflag(T), 0.0, 0, 0)
+_op(r::LifeRule, T) = SthOp(5, 0, 0.0, r.birth, r.survive)
+_op(l::Laplacian, T) = SthOp(6, _accflag(T), l.alpha, 0, 0)
+_op(f, T) = nothing
+_accflag(::Type{Float16}) = STH_OPF_ACC_F32
+_accflag(::Type) = UInt32(0)
+
+const AccelOp = Union{typeof(sum),typeof(mean),typeof(minimum),typeof(maximum),
+                      typeof(kernelproduct),LifeRule,Laplacian}
+
+# ------------------------------------------------------------------ context
+const _ctx = Ref{Ptr{Cvoid}}(C_NULL)
+
+sth_available() = isfile(libsth)
+
+function _context()
+    if _ctx[] == C_NULL
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        st = ccall((:sth_ctx_create, libsth), Cint, (Cint, Ref{Ptr{Cvoid}}), 0, h)
+        st == STH_OK || _throw(st)
+        _ctx[] = h[]
+    end
+    return _ctx[]
+end
+
+_lasterror() = unsafe_string(ccall((:sth_last_error, libsth), Cstring, ()))
+function _throw(st)
+    msg = _lasterror()
+    st == STH_EINVAL && throw(ArgumentError(msg))
+    error("libstencils_hip status $st: $msg")
+end
+
+# ------------------------------------------------------------ descriptors
+function _array(p::ROCArray{T,N}, logical::NTuple{N,Int}, halo::Int) where {T,N}
+    sz = ntuple(d -> d <= N ? logical[d] : 1, 3)
+    sd = ntuple(d -> d <= N ? stride(p, d) : 0, 3)
+    SthArray(Ptr{Cvoid}(UInt(pointer(p))), DTYPES[T], N, sz, sd, halo, 0)
+end
+
+_halo(A::StencilArray) = padding(A) isa Halo ? radius(stencil(A)) : 0
+
+function _stencil_desc(st::Stencil{R,N,L}) where {R,N,L}
+    base = st isa AbstractKernelStencil ? stencil(st) : st
+    offs = Int32[o[d] for o in offsets(typeof(base)) for d in 1:N]
+    return offs, Int32(0), Int32(R), Int32(N), Int32(L)
+end
+
+_weights(st::AbstractKernelStencil, T) = collect(T, kernel(st))
+_weights(::Stencil, T) = nothing
+
+_padval(b::Remove, ::Type{T}) where T = b.padval isa T ? Ref{T}(b.padval) : nothing
+_padval(b, ::Type{T}) where T = Ref{T}(zero(T))
+
+# ------------------------------------------------------ the accelerated call
+function _map(f, dest_parent::ROCArray, dest_halo::Int, src::StencilArray{R,T}) where {R,T}
+    haskey(DTYPES, T) || return STH_ENOTSUP
+    op = _op(f, T)
+    op === nothing && return STH_ENOTSUP
+    pv = _padval(boundary(src), T)
+    pv === nothing && return STH_ENOTSUP   # padval of another type promotes (src/gatherstencil.jl:50)
+    offs, kind, r, n, l = _stencil_desc(stencil(src))
+    w = _weights(stencil(src), T)
+    GC.@preserve offs w pv begin
+        sd = SthStencil(kind, r, n, l, pointer(offs),
+                        w === nothing ? C_NULL : Ptr{Cvoid}(pointer(w)),
+                        w === nothing ? Int32(-1) : DTYPES[eltype(w)], 0)
+        sa = _array(parent(src), size(src), _halo(src))
+        da = _array(dest_parent, size(src), dest_halo)
+        return ccall((:sth_map, libsth), Cint,
+                     (Ptr{Cvoid}, Ref{SthArray}, Ref{SthArray}, Ref{SthStencil}, Int32,
+                      Ptr{Cvoid}, Int32, Ref{SthOp}, UInt32, Ptr{Cvoid}),
+                     _context(), sa, da, sd, _bc(boundary(src)),
+                     Base.unsafe_convert(Ptr{Cvoid}, pv), _pad(padding(src)), op,
+                     UInt32(0), C_NULL)
+    end
+end
+
+# dest a plain device array of the logical size
+function Stencils.gatherstencil!(f::AccelOp, dest::ROCArray,
+                                 source::StencilArray{<:Any,<:Any,<:Any,<:ROCArray})
+    st = _map(f, dest, 0, source)
+    st == STH_OK && return dest
+    st == STH_ENOTSUP || _throw(st)
+    return invoke(Stencils.gatherstencil!,
+                  Tuple{Any,Any,AbstractStencilArray}, f, dest, source)
+end
+
+# dest a StencilArray over a device array (the SwitchingStencilArray step)
+function Stencils.gatherstencil!(f::AccelOp,
+                                 dest::StencilArray{<:Any,<:Any,<:Any,<:ROCArray},
+                                 source::StencilArray{<:Any,<:Any,<:Any,<:ROCArray})
+    st = _map(f, parent(dest), _halo(dest), source)
+    st == STH_OK && return dest
+    st == STH_ENOTSUP || _throw(st)
+    return invoke(Stencils.gatherstencil!,
+                  Tuple{Any,Any,AbstractStencilArray}, f, dest, source)
+end
+
+"""
+    mapstencil_steps!(f, A::SwitchingStencilArray, steps; graph=true)
+
+`steps` calls of `A = mapstencil!(f, A)`, ping-ponged on device and, with
+`graph`, replayed from a captured hipGraph. Returns the switched array.
+"""
+function mapstencil_steps!(f::AccelOp, A::SwitchingStencilArray{R,T,N,<:ROCArray},
+                           steps::Integer; graph::Bool=true) where {R,T,N}
+    op = _op(f, T)
+    pv = _padval(boundary(A), T)
+    (op === nothing || pv === nothing) && return _fallback_steps(f, A, steps)
+    offs, kind, r, n, l = _stencil_desc(stencil(A))
+    w = _weights(stencil(A), T)
+    h = padding(A) isa Halo ? R : 0
+    lsz = size(source(A)) .- 2h
+    cur = Ref{Int32}(0)
+    st = GC.@preserve offs w pv begin
+        sd = SthStencil(kind, r, n, l, pointer(offs),
+                        w === nothing ? C_NULL : Ptr{Cvoid}(pointer(w)),
+                        w === nothing ? Int32(-1) : DTYPES[eltype(w)], 0)
+        a = _array(source(A), lsz, h)
+        b = _array(dest(A), lsz, h)
+        ccall((:sth_iterate, libsth), Cint,
+              (Ptr{Cvoid}, Ref{SthArray}, Ref{SthArray}, Ref{SthStencil}, Int32, Ptr{Cvoid},
+               Int32, Ref{SthOp}, Int64, UInt32, Ref{Int32}, Ptr{Cvoid}),
+              _context(), a, b, sd, _bc(boundary(A)), Base.unsafe_convert(Ptr{Cvoid}, pv),
+              _pad(padding(A)), op, Int64(steps),
+              graph ? STH_GRAPH : UInt32(0), cur, C_NULL)
+    end
+    st == STH_ENOTSUP && return _fallback_steps(f, A, steps)
+    st == STH_OK || _throw(st)
+    return cur[] == 0 ? A : switch(A)
+end
+
+function _fallback_steps(f, A, steps)
+    for _ in 1:steps
+        A = Stencils.gatherstencil!(f, A)
+    end
+    return A
+end
+
+end # module
