@@ -142,6 +142,7 @@ typedef struct sth_op {
 #define STH_ASYNC 1u     /* return without synchronising the stream */
 #define STH_GRAPH 2u     /* sth_iterate: replay the steps as a captured hipGraph */
 #define STH_GENERIC 4u   /* force the runtime-offset kernel (testing / tuning) */
+#define STH_NO_TUNE 16u  /* use the default kernel variant; never time variants */
 
 typedef struct sth_ctx sth_ctx;
 
@@ -192,6 +193,31 @@ int sth_iterate(sth_ctx* ctx, const sth_array* a, const sth_array* b,
                 int32_t padding, const sth_op* op, int64_t steps,
                 uint32_t flags, int32_t* current, void* stream);
 
+/*
+ * Autotuner.  Several kernel variants can run one call (streaming kernels with
+ * different rows per wave, the LDS-tiled kernel, Life rows per wave); all of
+ * them compute identical results.  sth_map / sth_iterate time the variants of
+ * a (shape, element type, op, size bucket) family the first time they see it
+ * -- unless STH_NO_TUNE is passed, STH_AUTOTUNE=0 is set in the environment or
+ * the stream is being captured -- and reuse the fastest afterwards.
+ * STH_TUNE_FILE names a file the decisions are loaded from and appended to.
+ * No counterpart in the reference (KernelAbstractions picks a fixed
+ * workgroup size, src/gatherstencil.jl:97-99).
+ *
+ * sth_autotune times every variant of this call now (replacing an earlier
+ * decision), runs the call with the winner and writes the winner's name into
+ * `chosen` (may be NULL).  sth_tuning_table writes "family\tvariant\n" lines;
+ * `needed` receives the buffer size the table requires.
+ */
+int sth_autotune(sth_ctx* ctx, const sth_array* src, const sth_array* dst,
+                 const sth_stencil* st, int32_t boundary, const void* padval,
+                 int32_t padding, const sth_op* op, void* stream, char* chosen,
+                 size_t chosen_len);
+int sth_tuning_table(sth_ctx* ctx, char* buf, size_t len, size_t* needed);
+/* Testing and tuning hook: while `pattern` is non-empty, every call uses the
+ * first variant whose name contains it (if any), bypassing the tuner. */
+int sth_set_variant(sth_ctx* ctx, const char* pattern);
+
 /* One mapstencil call of a batch (sth_map_batch). */
 typedef struct sth_job {
   sth_array src;
@@ -207,8 +233,9 @@ typedef struct sth_job {
 
 /*
  * Many independent gatherstencil!(f, dest, source) calls -- mixed sizes,
- * shapes and reductions -- in one persistent launch per element type.  Jobs
- * outside the batched catalogue (and Halo jobs that need a refresh) run as
+ * shapes and reductions -- in one persistent launch per element type
+ * (workgroups pull tiles from a device-side counter).  Halo jobs get their
+ * update_boundary! refresh first; jobs outside the batched catalogue run as
  * individual launches inside the call.  Each result is identical to an
  * sth_map call on that job alone.  No counterpart in the reference, which
  * runs one KernelAbstractions launch per mapstencil call
@@ -217,6 +244,81 @@ typedef struct sth_job {
  */
 int sth_map_batch(sth_ctx* ctx, int32_t njobs, const sth_job* jobs, uint32_t flags,
                   void* stream);
+
+/*
+ * Serving loop: continuous batching of iterated stencil solves.
+ *
+ * A request is `steps` calls of mapstencil!(f, A::SwitchingStencilArray)
+ * (src/gatherstencil.jl:77-83) on its own state.  The server keeps the two
+ * buffers of every admitted request resident in an arena of `hbm_budget`
+ * bytes of device memory, admits requests in arrival order as soon as their
+ * arrival time has passed and their buffers fit (and fewer than `max_active`
+ * run), and advances every admitted request together: each step of a round
+ * is one persistent launch over all of them (sth_map_batch's kernel), so
+ * requests that arrive while others run join at the next round.  Finished
+ * states are copied to the request's `result` and their buffers freed.
+ * STH_SERVER_SERIAL instead serves one request at a time with one plain
+ * launch per step (the unbatched path, for comparison).
+ * No counterpart in the reference; callers: a serving front end (the
+ * Python `Server`, a Julia task) that receives solve requests.
+ */
+typedef struct sth_server sth_server;
+
+#define STH_SERVER_SERIAL 1u
+
+typedef struct sth_server_config {
+  size_t hbm_budget;         /* bytes of device memory for request buffers */
+  int32_t max_active;        /* most requests admitted at once (0: no limit) */
+  int32_t round_steps;       /* steps per scheduling round (0: 8) */
+  uint32_t flags;            /* STH_SERVER_* */
+  int32_t reserved;
+} sth_server_config;
+
+typedef struct sth_request {
+  sth_array state;      /* initial state: the PARENT layout, dense (stride[1] = size[0] + 2*halo,
+                           ...); data is host or device memory; copied at admission */
+  sth_stencil stencil;  /* copied at submit */
+  sth_op op;
+  const void* padval;   /* copied at submit */
+  int32_t boundary;
+  int32_t padding;
+  int64_t steps;
+  void* result;         /* host or device memory, same layout as state: the final state */
+  double arrival;       /* server-clock seconds before which the request is not admitted */
+  uint64_t tag;         /* caller's identifier, returned in the completion */
+} sth_request;
+
+typedef struct sth_completion {
+  uint64_t tag;
+  int32_t status;       /* STH_OK or the failure of this request */
+  int32_t reserved;
+  double arrival;       /* server-clock seconds */
+  double admitted;
+  double finished;      /* when the result copy completed on the device */
+  int64_t steps;
+  int64_t cells;        /* logical cells of the state */
+} sth_completion;
+
+typedef struct sth_server_stats {
+  int64_t submitted, completed, active, pending;
+  int64_t rounds, launches;
+  int64_t cell_steps;    /* completed requests' cells x steps */
+  size_t hbm_in_use, hbm_peak;
+  double started;        /* wall seconds since sth_server_start */
+} sth_server_stats;
+
+int sth_server_create(sth_ctx* ctx, const sth_server_config* cfg, void* stream, sth_server** out);
+/* Validates and queues a request (any time before or after start). */
+int sth_server_submit(sth_server* srv, const sth_request* req);
+/* Starts the server clock (arrival 0) and the scheduler thread. */
+int sth_server_start(sth_server* srv);
+/* Copies up to `max` completions not yet returned; waits up to wait_ms for one. */
+int sth_server_poll(sth_server* srv, sth_completion* out, int32_t max, int32_t* n, int32_t wait_ms);
+/* Waits until every submitted request has completed (or timeout_s passes: STH_EDEVICE). */
+int sth_server_drain(sth_server* srv, double timeout_s);
+int sth_server_stats_get(sth_server* srv, sth_server_stats* out);
+/* Stops the scheduler (after draining) and frees the arena. */
+int sth_server_destroy(sth_server* srv);
 
 #ifdef __cplusplus
 }

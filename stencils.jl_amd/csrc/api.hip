@@ -92,10 +92,31 @@ struct Plan {
   LaunchFn fn_steady;  // valid when the source was written by the previous step
   const char* kname;
   int32_t halo;
-  int32_t pad_;
+  int32_t bvariant;  // case of the batched kernel for this call, -1 if none
+  int32_t bsv, bsky; // its streaming tile (64*bsv cells x 4*bsky rows)
+  int32_t dtype;     // sth_dtype of the source
   HaloParams hp;
   void (*halo_fn)(const HaloParams&, hipStream_t);
 };
+
+// One launchable kernel choice for a plan; make_plan lists every variant that
+// can run the call (the first is the default) and the autotuner picks one.
+struct Variant {
+  LaunchFn fn = nullptr;
+  LaunchFn fn_steady = nullptr;
+  const char* name = "";
+  int32_t tiles_x = 0, tiles_y = 0, tiles_z = 0;
+};
+
+void apply_variant(Plan& pl, const Variant& v) {
+  pl.fn = v.fn;
+  pl.fn_steady = v.fn_steady;
+  pl.kname = v.name;
+  pl.p.tiles_x = v.tiles_x;
+  pl.p.tiles_y = v.tiles_y;
+  pl.p.tiles_z = v.tiles_z;
+  if (pl.p.src.n0 * pl.p.src.n1 * pl.p.src.n2 == 0) pl.fn = pl.fn_steady = nullptr;  // nothing to launch
+}
 
 DevArr make_devarr(const sth_array& a, bool with_halo) {
   DevArr d{};
@@ -147,7 +168,8 @@ bool offsets_equal(const int8_t* tbl, int L, const std::vector<int>& offs3) {
 }
 
 int make_plan(const sth_array* src, const sth_array* dst, const sth_stencil* st, int32_t bc,
-              const void* padval, int32_t padding, const sth_op* op, uint32_t flags, Plan* out) {
+              const void* padval, int32_t padding, const sth_op* op, uint32_t flags, Plan* out,
+              std::vector<Variant>* cands_out = nullptr, std::string* group_out = nullptr) {
   int rc;
   if ((rc = check_array(src, "source"))) return rc;
   if ((rc = check_array(dst, "dest"))) return rc;
@@ -271,78 +293,127 @@ int make_plan(const sth_array* src, const sth_array* dst, const sth_stencil* st,
   const auto aligned16 = [](const DevArr& d) {
     return (reinterpret_cast<uintptr_t>(d.base) % 16 == 0) && (d.s1 % 16 == 0);
   };
+  std::vector<Variant> cands;
+  std::string group;
+  const int64_t cells = n[0] * n[1] * n[2];
+  auto ilog2 = [](int64_t v) { int r = 0; while (v > 1) { v >>= 1; ++r; } return r; };
+  const std::string bucket = "|c" + std::to_string(ilog2(cells)) + "|x" + std::to_string(ilog2(n[0]));
   if (!(flags & STH_GENERIC) && op->kind == STH_OP_LIFE && (sdt == STH_U8 || sdt == STH_BOOL) &&
       moore1 && src->ndim == 2 && n[0] % 16 == 0 && aligned16(p.src) && aligned16(p.dst)) {
-    // rows per wave: a tuning knob (STH_LIFE_KY overrides the default)
-    int KY = 4;
-    if (const char* e = std::getenv("STH_LIFE_KY")) KY = std::atoi(e);
-    if (KY != 2 && KY != 4 && KY != 8 && KY != 16) KY = 4;
-    p.tiles_x = static_cast<int32_t>((n[0] + 1023) / 1024);
-    p.tiles_y = static_cast<int32_t>((n[1] + 4 * KY - 1) / (4 * KY));
-    p.tiles_z = 1;
     const bool conway = op->birth_mask == (1ull << 3) && op->survive_mask == ((1ull << 2) | (1ull << 3));
-    auto pick = [&](bool norm) -> LaunchFn {
-      if (conway && norm)
-        return KY == 2 ? &launch_life<2, true, true> : KY == 4 ? &launch_life<4, true, true>
-             : KY == 16 ? &launch_life<16, true, true> : &launch_life<8, true, true>;
-      if (conway)
-        return KY == 2 ? &launch_life<2, true, false> : KY == 4 ? &launch_life<4, true, false>
-             : KY == 16 ? &launch_life<16, true, false> : &launch_life<8, true, false>;
-      if (norm)
-        return KY == 2 ? &launch_life<2, false, true> : KY == 4 ? &launch_life<4, false, true>
-             : KY == 16 ? &launch_life<16, false, true> : &launch_life<8, false, true>;
-      return KY == 2 ? &launch_life<2, false, false> : KY == 4 ? &launch_life<4, false, false>
-           : KY == 16 ? &launch_life<16, false, false> : &launch_life<8, false, false>;
-    };
-    pl.fn = pick(true);
     // Every byte this kernel writes is 0 or 1, so later steps of an iteration
     // can skip normalising the source -- unless they read a halo (Use).
-    pl.fn_steady = (halo_pad && bc == STH_USE) ? pl.fn : pick(false);
-    pl.kname = "life_swar";
+    const bool steady_norm = halo_pad && bc == STH_USE;
+    auto add = [&](int ky, LaunchFn norm, LaunchFn raw, const char* name) {
+      Variant v;
+      v.fn = norm;
+      v.fn_steady = steady_norm ? norm : raw;
+      v.name = name;
+      v.tiles_x = static_cast<int32_t>((n[0] + 1023) / 1024);
+      v.tiles_y = static_cast<int32_t>((n[1] + 4 * ky - 1) / (4 * ky));
+      v.tiles_z = 1;
+      cands.push_back(v);
+    };
+    // rows per wave: 4 is the default; the autotuner may pick another
+    if (conway) {
+      add(4, &launch_life<4, true, true>, &launch_life<4, true, false>, "life_conway/ky4");
+      add(2, &launch_life<2, true, true>, &launch_life<2, true, false>, "life_conway/ky2");
+      add(8, &launch_life<8, true, true>, &launch_life<8, true, false>, "life_conway/ky8");
+      add(16, &launch_life<16, true, true>, &launch_life<16, true, false>, "life_conway/ky16");
+    } else {
+      add(4, &launch_life<4, false, true>, &launch_life<4, false, false>, "life_table/ky4");
+      add(2, &launch_life<2, false, true>, &launch_life<2, false, false>, "life_table/ky2");
+      add(8, &launch_life<8, false, true>, &launch_life<8, false, false>, "life_table/ky8");
+      add(16, &launch_life<16, false, true>, &launch_life<16, false, false>, "life_table/ky16");
+    }
+    group = std::string(conway ? "life_conway" : "life_table") + (sdt == STH_BOOL ? "_bool" : "_u8") + bucket;
     if (halo_pad && bc != STH_USE) {
       // map boundaries in registers and refresh the source halo in the same launch
       p.read_halo = 0;
       p.fuse_halo = 1;
     }
   } else {
-    const KernelEntry* pick = nullptr;
     if (!(flags & STH_GENERIC)) {
+      std::string family;  // shapes with equal offsets (e.g. Window(1), Circle(1)) share kernels
       for (const auto& e : registry()) {
         if (e.L == 0 || e.dtype != sdt || e.op != op->kind || e.L != L || e.ndim != st->ndim) continue;
-        if (offsets_equal(e.offs, L, offs3)) {
-          pick = &e;
-          break;
+        if (!offsets_equal(e.offs, L, offs3)) continue;
+        const char* slash = std::strchr(e.name, '/');
+        const std::string fam(e.name, slash ? static_cast<size_t>(slash - e.name) : std::strlen(e.name));
+        if (family.empty()) family = fam;
+        else if (fam != family) continue;
+        Variant v;
+        v.fn = v.fn_steady = e.fn;
+        v.name = e.name;
+        if (e.cfg == CFG_STREAM) {
+          v.tiles_x = static_cast<int32_t>((n[0] + 64 * e.sv - 1) / (64 * e.sv));
+          v.tiles_y = static_cast<int32_t>((n[1] + 4 * e.sky - 1) / (4 * e.sky));
+          if (n[2] > (int64_t(1) << 31) / 2) continue;
+          v.tiles_z = static_cast<int32_t>(n[2]);
+        } else {
+          const TileDims td = tile_dims(e.cfg);
+          const int staged = (sdt == STH_BF16 || sdt == STH_F16) ? 4 : dtype_size(sdt);
+          const int64_t lds = static_cast<int64_t>(td.tx + 2 * reach[0]) * (td.ty + 2 * reach[1]) *
+                              (td.tz + 2 * reach[2]) * staged;
+          if (lds > 65536) continue;
+          v.tiles_x = static_cast<int32_t>((n[0] + td.tx - 1) / td.tx);
+          v.tiles_y = static_cast<int32_t>((n[1] + td.ty - 1) / td.ty);
+          v.tiles_z = static_cast<int32_t>((n[2] + td.tz - 1) / td.tz);
         }
+        if (static_cast<int64_t>(v.tiles_x) * v.tiles_y * v.tiles_z > (int64_t(1) << 31) - 1) continue;
+        cands.push_back(v);
+      }
+      if (!cands.empty()) {
+        const char* slash = std::strchr(cands[0].name, '/');
+        group.assign(cands[0].name, slash ? static_cast<size_t>(slash - cands[0].name) : std::strlen(cands[0].name));
+        group += bucket;
       }
     }
-    if (!pick) {
+    if (cands.empty()) {
       if (L > kMaxRuntimeL) return fail(STH_ENOTSUP, "stencil with %d offsets (runtime max %d)", L, kMaxRuntimeL);
       const int cfg = reach[2] > 0 ? CFG_RT3D : (n[1] == 1 && reach[1] == 0 ? CFG_RT1D : CFG_RT2D);
+      const KernelEntry* pick = nullptr;
       for (const auto& e : registry())
         if (e.L == 0 && e.dtype == sdt && e.op == op->kind && e.cfg == cfg) {
           pick = &e;
           break;
         }
+      if (!pick) return fail(STH_ENOTSUP, "no kernel for dtype %d op %d", sdt, op->kind);
       for (int k = 0; k < L; ++k)
         p.offs[k] = (offs3[3 * k] & 0xff) | ((offs3[3 * k + 1] & 0xff) << 8) | ((offs3[3 * k + 2] & 0xff) << 16);
+      const TileDims td = tile_dims(pick->cfg);
+      const int staged = (sdt == STH_BF16 || sdt == STH_F16) ? 4 : dtype_size(sdt);
+      const int64_t lds = static_cast<int64_t>(td.tx + 2 * reach[0]) * (td.ty + 2 * reach[1]) *
+                          (td.tz + 2 * reach[2]) * staged;
+      if (lds > 65536)
+        return fail(STH_ENOTSUP, "stencil reach needs %lld bytes of LDS per tile", static_cast<long long>(lds));
+      const int64_t tx = (n[0] + td.tx - 1) / td.tx, ty = (n[1] + td.ty - 1) / td.ty,
+                    tz = (n[2] + td.tz - 1) / td.tz;
+      if (tx * ty * tz > (int64_t(1) << 31) - 1) return fail(STH_ENOTSUP, "array too large for one launch");
+      Variant v;
+      v.fn = v.fn_steady = pick->fn;
+      v.name = pick->name;
+      v.tiles_x = static_cast<int32_t>(tx);
+      v.tiles_y = static_cast<int32_t>(ty);
+      v.tiles_z = static_cast<int32_t>(tz);
+      cands.push_back(v);
     }
-    if (!pick) return fail(STH_ENOTSUP, "no kernel for dtype %d op %d", sdt, op->kind);
-    const TileDims td = tile_dims(pick->cfg);
-    const int staged = (sdt == STH_BF16 || sdt == STH_F16) ? 4 : dtype_size(sdt);
-    const int64_t lds = static_cast<int64_t>(td.tx + 2 * reach[0]) * (td.ty + 2 * reach[1]) *
-                        (td.tz + 2 * reach[2]) * staged;
-    if (lds > 65536) return fail(STH_ENOTSUP, "stencil reach needs %lld bytes of LDS per tile", static_cast<long long>(lds));
-    const int64_t tx = (n[0] + td.tx - 1) / td.tx, ty = (n[1] + td.ty - 1) / td.ty,
-                  tz = (n[2] + td.tz - 1) / td.tz;
-    if (tx * ty * tz > (int64_t(1) << 31) - 1) return fail(STH_ENOTSUP, "array too large for one launch");
-    p.tiles_x = static_cast<int32_t>(tx);
-    p.tiles_y = static_cast<int32_t>(ty);
-    p.tiles_z = static_cast<int32_t>(tz);
-    pl.fn = pick->fn;
-    pl.fn_steady = pick->fn;
-    pl.kname = pick->name;
   }
-  if (n[0] * n[1] * n[2] == 0) pl.fn = pl.fn_steady = nullptr;  // empty array: nothing to launch
+  apply_variant(pl, cands[0]);
+  if (cands_out) *cands_out = std::move(cands);
+
+  // the batched kernel's variant for this (dtype, op, offsets), if any
+  pl.dtype = sdt;
+  pl.bvariant = -1;
+  for (const auto& bv : batch_variants()) {
+    if (bv.dtype != sdt || bv.op != op->kind || bv.L != L || bv.ndim != st->ndim) continue;
+    if (!offsets_equal(bv.offs, L, offs3)) continue;
+    pl.bvariant = bv.variant;
+    pl.bsv = bv.sv;
+    pl.bsky = bv.sky;
+    break;
+  }
+  if (group_out) *group_out = group;
 
   // ---- halo refresh (update_boundary!) --------------------------------
   if (halo_pad && bc != STH_USE && src->halo > 0 && !p.fuse_halo) {
@@ -394,14 +465,21 @@ struct BatchSlot {
   bool used = false;
 };
 
+constexpr int kBatchSlots = 8;  // job tables in flight (a launch reuses a slot once it has finished)
+
 struct sth_ctx {
   int device = 0;
   int cus = 0;
   hipStream_t capture = nullptr;  // private stream used only for graph capture
   std::mutex mu;
   std::map<std::string, hipGraphExec_t> graphs;
-  BatchSlot slots[4];
+  BatchSlot slots[kBatchSlots];
   unsigned next_slot = 0;
+  // autotuner: shape family + size bucket -> chosen variant name
+  std::map<std::string, std::string> tuned;
+  bool autotune = true;    // STH_AUTOTUNE=0 disables timing (defaults are used)
+  std::string forced;      // sth_set_variant: pick the first variant whose name contains it
+  std::string tune_file;   // STH_TUNE_FILE: decisions are loaded from and appended to it
 };
 
 using namespace sth;
@@ -421,6 +499,175 @@ int set_device(sth_ctx* ctx) {
   if (!ctx) return fail(STH_EINVAL, "context is NULL");
   STH_HIP(hipSetDevice(ctx->device));
   return STH_OK;
+}
+
+// Autotuner: the variants of a call compute identical results (same fold
+// expressions per cell), so the fastest may be chosen freely.  Each variant
+// runs once to warm up and three times between events; the fastest average
+// wins and is remembered per (shape family, element type, op, size bucket).
+// Timing is skipped while the stream is being captured, with STH_NO_TUNE,
+// or when `lookup_only` is set; the default (first) variant is used then.
+int tune_pick(sth_ctx* ctx, const Plan& pl, const std::vector<Variant>& c, const std::string& group,
+              uint32_t flags, hipStream_t s, bool force, bool lookup_only, int* idx) {
+  *idx = 0;
+  if (c.size() <= 1 || group.empty()) return STH_OK;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (!ctx->forced.empty()) {
+      for (size_t i = 0; i < c.size(); ++i)
+        if (std::strstr(c[i].name, ctx->forced.c_str())) {
+          *idx = static_cast<int>(i);
+          return STH_OK;
+        }
+    }
+  }
+  if (!force) {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->tuned.find(group);
+    if (it != ctx->tuned.end()) {
+      for (size_t i = 0; i < c.size(); ++i)
+        if (it->second == c[i].name) {
+          *idx = static_cast<int>(i);
+          return STH_OK;
+        }
+    }
+  }
+  if (lookup_only || (flags & STH_NO_TUNE) || (!ctx->autotune && !force)) return STH_OK;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  STH_HIP(hipStreamIsCapturing(s, &cs));
+  if (cs != hipStreamCaptureStatusNone) return STH_OK;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  STH_HIP(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(STH_EDEVICE, "hipEventCreate failed");
+  }
+  int rc = STH_OK;
+  float best = 0.0f;
+  int bi = 0;
+  for (size_t i = 0; i < c.size() && rc == STH_OK; ++i) {
+    Plan t = pl;
+    apply_variant(t, c[i]);
+    rc = run_plan(t, s);
+    if (rc == STH_OK && hipEventRecord(e0, s) != hipSuccess) rc = fail(STH_EDEVICE, "hipEventRecord failed");
+    for (int r = 0; r < 3 && rc == STH_OK; ++r) rc = run_plan(t, s);
+    if (rc == STH_OK && hipEventRecord(e1, s) != hipSuccess) rc = fail(STH_EDEVICE, "hipEventRecord failed");
+    if (rc == STH_OK && hipEventSynchronize(e1) != hipSuccess) rc = fail(STH_EDEVICE, "hipEventSynchronize failed");
+    float ms = 0.0f;
+    if (rc == STH_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = fail(STH_EDEVICE, "hipEventElapsedTime failed");
+    if (rc == STH_OK && (i == 0 || ms < best)) {
+      best = ms;
+      bi = static_cast<int>(i);
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  *idx = bi;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ctx->tuned[group] = c[bi].name;
+  if (!ctx->tune_file.empty()) {
+    if (FILE* f = std::fopen(ctx->tune_file.c_str(), "a")) {
+      std::fprintf(f, "%s\t%s\n", group.c_str(), c[bi].name);
+      std::fclose(f);
+    }
+  }
+  return STH_OK;
+}
+
+// One step of many independent calls: calls with a batched variant go into
+// one persistent launch per element type (after their halo refresh), the
+// rest run as single launches first.  `serial_only` runs every call alone.
+int launch_batched(sth_ctx* ctx, const Plan* const* plans, int32_t n, bool serial_only, hipStream_t s) {
+  int rc = STH_OK;
+  std::map<int, std::vector<BatchJob>> groups;
+  for (int32_t i = 0; i < n; ++i) {
+    const Plan& pl = *plans[i];
+    if (serial_only || pl.bvariant < 0 || !pl.fn) {
+      if (pl.fn || pl.halo)
+        if ((rc = run_plan(pl, s))) return rc;
+      continue;
+    }
+    if (pl.halo) {  // update_boundary! before the batched gather
+      pl.halo_fn(pl.hp, s);
+      STH_HIP(hipGetLastError());
+    }
+    BatchJob bj;
+    std::memset(static_cast<void*>(&bj), 0, sizeof bj);
+    bj.src = pl.p.src;
+    bj.dst = pl.p.dst;
+    bj.bc = pl.p.bc;
+    bj.read_halo = pl.p.read_halo;
+    bj.padval = pl.p.padval;
+    bj.op = pl.p.op;
+    bj.variant = pl.bvariant;
+    bj.tiles_x = static_cast<int32_t>((pl.p.src.n0 + 64 * pl.bsv - 1) / (64 * pl.bsv));
+    bj.tiles_y = static_cast<int32_t>((pl.p.src.n1 + 4 * pl.bsky - 1) / (4 * pl.bsky));
+    bj.tiles_z = static_cast<int32_t>(pl.p.src.n2);
+    groups[pl.dtype].push_back(bj);
+  }
+  constexpr size_t kJobsOffset = 64;  // the tile counter sits in front of the job table
+  for (auto& kv : groups) {
+    const BatchKernelEntry* ke = nullptr;
+    for (const auto& e : batch_kernels())
+      if (e.dtype == kv.first) ke = &e;
+    if (!ke) return fail(STH_EDEVICE, "no batched kernel for dtype %d", kv.first);
+    std::vector<BatchJob>& bj = kv.second;
+    int64_t total = 0;
+    for (auto& j : bj) {
+      j.tile_begin = total;
+      total += static_cast<int64_t>(j.tiles_x) * j.tiles_y * j.tiles_z;
+    }
+    if (total == 0) continue;
+    if (total > (int64_t(1) << 31)) return fail(STH_ENOTSUP, "batch of %lld tiles is too large", static_cast<long long>(total));
+    const size_t bytes = kJobsOffset + bj.size() * sizeof(BatchJob);
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    BatchSlot& sl = ctx->slots[ctx->next_slot++ % kBatchSlots];
+    if (sl.used) STH_HIP(hipEventSynchronize(sl.done));  // its last launch has finished with it
+    if (sl.bytes < bytes) {
+      if (sl.host) STH_HIP(hipHostFree(sl.host));
+      if (sl.dev) STH_HIP(hipFree(sl.dev));
+      sl.host = sl.dev = nullptr;
+      sl.bytes = 0;
+      const size_t cap = std::max<size_t>(bytes, 16384);
+      STH_HIP(hipHostMalloc(&sl.host, cap, hipHostMallocDefault));
+      STH_HIP(hipMalloc(&sl.dev, cap));
+      sl.bytes = cap;
+    }
+    if (!sl.done) STH_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+    std::memset(sl.host, 0, kJobsOffset);  // the counter starts at zero
+    std::memcpy(static_cast<char*>(sl.host) + kJobsOffset, bj.data(), bj.size() * sizeof(BatchJob));
+    STH_HIP(hipMemcpyAsync(sl.dev, sl.host, bytes, hipMemcpyHostToDevice, s));
+    BatchParams bp;
+    bp.jobs = reinterpret_cast<const BatchJob*>(static_cast<char*>(sl.dev) + kJobsOffset);
+    bp.queue = static_cast<unsigned int*>(sl.dev);
+    bp.njobs = static_cast<int32_t>(bj.size());
+    bp.pad_ = 0;
+    bp.total_tiles = total;
+    const int64_t resident = static_cast<int64_t>(ke->occupancy()) * ctx->cus;
+    const unsigned blocks = static_cast<unsigned>(std::min<int64_t>(total, resident));
+    ke->fn(bp, blocks, s);
+    STH_HIP(hipGetLastError());
+    STH_HIP(hipEventRecord(sl.done, s));
+    sl.used = true;
+  }
+  return STH_OK;
+}
+
+void load_tune_file(sth_ctx* ctx) {
+  if (ctx->tune_file.empty()) return;
+  FILE* f = std::fopen(ctx->tune_file.c_str(), "r");
+  if (!f) return;
+  char line[512];
+  while (std::fgets(line, sizeof line, f)) {
+    char* tab = std::strchr(line, '\t');
+    if (!tab) continue;
+    *tab = 0;
+    char* name = tab + 1;
+    name[std::strcspn(name, "\r\n")] = 0;
+    ctx->tuned[line] = name;
+  }
+  std::fclose(f);
 }
 }  // namespace
 
@@ -446,6 +693,9 @@ int sth_ctx_create(int device, sth_ctx** out) {
     delete c;
     return fail(STH_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
   }
+  if (const char* v = std::getenv("STH_AUTOTUNE")) c->autotune = std::strcmp(v, "0") != 0;
+  if (const char* v = std::getenv("STH_TUNE_FILE")) c->tune_file = v;
+  load_tune_file(c);
   *out = c;
   g_err.clear();
   return STH_OK;
@@ -564,10 +814,61 @@ int sth_map(sth_ctx* ctx, const sth_array* src, const sth_array* dst, const sth_
   int rc = set_device(ctx);
   if (rc) return rc;
   Plan pl;
-  if ((rc = make_plan(src, dst, st, boundary, padval, padding, op, flags, &pl))) return rc;
+  std::vector<Variant> cands;
+  std::string group;
+  if ((rc = make_plan(src, dst, st, boundary, padval, padding, op, flags, &pl, &cands, &group))) return rc;
   auto s = static_cast<hipStream_t>(stream);
+  int idx = 0;
+  if ((rc = tune_pick(ctx, pl, cands, group, flags, s, false, false, &idx))) return rc;
+  apply_variant(pl, cands[idx]);
   if ((rc = run_plan(pl, s))) return rc;
   if (!(flags & STH_ASYNC)) STH_HIP(hipStreamSynchronize(s));
+  return STH_OK;
+  STH_GUARD_END
+}
+
+int sth_autotune(sth_ctx* ctx, const sth_array* src, const sth_array* dst, const sth_stencil* st,
+                 int32_t boundary, const void* padval, int32_t padding, const sth_op* op, void* stream,
+                 char* chosen, size_t chosen_len) {
+  STH_GUARD_BEGIN
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  Plan pl;
+  std::vector<Variant> cands;
+  std::string group;
+  if ((rc = make_plan(src, dst, st, boundary, padval, padding, op, 0, &pl, &cands, &group))) return rc;
+  auto s = static_cast<hipStream_t>(stream);
+  int idx = 0;
+  if ((rc = tune_pick(ctx, pl, cands, group, 0, s, true, false, &idx))) return rc;
+  apply_variant(pl, cands[idx]);
+  if ((rc = run_plan(pl, s))) return rc;
+  STH_HIP(hipStreamSynchronize(s));
+  if (chosen && chosen_len) {
+    std::snprintf(chosen, chosen_len, "%s", cands[idx].name);
+  }
+  return STH_OK;
+  STH_GUARD_END
+}
+
+int sth_set_variant(sth_ctx* ctx, const char* pattern) {
+  STH_GUARD_BEGIN
+  if (!ctx) return fail(STH_EINVAL, "context is NULL");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ctx->forced = pattern ? pattern : "";
+  return STH_OK;
+  STH_GUARD_END
+}
+
+int sth_tuning_table(sth_ctx* ctx, char* buf, size_t len, size_t* needed) {
+  STH_GUARD_BEGIN
+  if (!ctx) return fail(STH_EINVAL, "context is NULL");
+  std::string out;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    for (const auto& kv : ctx->tuned) out += kv.first + "\t" + kv.second + "\n";
+  }
+  if (needed) *needed = out.size() + 1;
+  if (buf && len) std::snprintf(buf, len, "%s", out.c_str());
   return STH_OK;
   STH_GUARD_END
 }
@@ -586,9 +887,18 @@ int sth_iterate(sth_ctx* ctx, const sth_array* a, const sth_array* b, const sth_
   if (op && op->kind == STH_OP_MEAN && !is_float(a->dtype))
     return fail(STH_EINVAL, "mapstencil! cannot store a Float64 mean into an integer buffer");
   Plan ab, ba;
-  if ((rc = make_plan(a, b, st, boundary, padval, padding, op, flags, &ab))) return rc;
-  if ((rc = make_plan(b, a, st, boundary, padval, padding, op, flags, &ba))) return rc;
+  std::vector<Variant> cab, cba;
+  std::string group;
+  if ((rc = make_plan(a, b, st, boundary, padval, padding, op, flags, &ab, &cab, &group))) return rc;
+  if ((rc = make_plan(b, a, st, boundary, padval, padding, op, flags, &ba, &cba))) return rc;
   auto s = static_cast<hipStream_t>(stream);
+  if (steps > 0) {
+    // choose the variant on the a -> b step (idempotent: it only writes b)
+    int idx = 0;
+    if ((rc = tune_pick(ctx, ab, cab, group, flags, s, false, false, &idx))) return rc;
+    apply_variant(ab, cab[idx]);
+    apply_variant(ba, cba[idx]);
+  }
   *current = static_cast<int32_t>(steps % 2);
   if (steps == 0) return STH_OK;
   // The first step runs eagerly (it may read data this library did not
@@ -653,103 +963,31 @@ int sth_map_batch(sth_ctx* ctx, int32_t njobs, const sth_job* jobs, uint32_t fla
   if (rc) return rc;
   if (njobs < 0 || (njobs > 0 && !jobs)) return fail(STH_EINVAL, "bad job list");
   auto s = static_cast<hipStream_t>(stream);
+  const bool serial_only = (flags & STH_BATCH_SERIAL) != 0;
   std::vector<Plan> plans(static_cast<size_t>(njobs));
+  std::vector<const Plan*> ptrs(static_cast<size_t>(njobs));
   for (int32_t i = 0; i < njobs; ++i) {
     const sth_job& jb = jobs[i];
+    std::vector<Variant> cands;
+    std::string group;
     if ((rc = make_plan(&jb.src, &jb.dst, &jb.stencil, jb.boundary, jb.padval, jb.padding, &jb.op,
-                        flags & ~STH_GENERIC, &plans[i]))) {
+                        flags & ~STH_GENERIC, &plans[i], &cands, &group))) {
       std::string m = g_err;
       return fail(rc, "job %d: %s", i, m.c_str());
     }
+    // one launch per job: the tuned single-array kernel (tuning may time it
+    // here); inside a batched call only decisions already made are used
+    int idx = 0;
+    if ((rc = tune_pick(ctx, plans[i], cands, group, flags, s, false, !serial_only, &idx))) return rc;
+    apply_variant(plans[i], cands[idx]);
+    ptrs[i] = &plans[i];
   }
-  // group the batchable jobs by element type
-  std::map<int, std::vector<BatchJob>> groups;
-  std::vector<int32_t> serial;
-  for (int32_t i = 0; i < njobs; ++i) {
-    const sth_job& jb = jobs[i];
-    const Plan& pl = plans[i];
-    const int dt = jb.src.dtype;
-    const BatchVariant* v = nullptr;
-    if (!(flags & STH_BATCH_SERIAL) && !pl.halo && pl.fn &&
-        (dt == STH_F32 || dt == STH_F64) && jb.op.kind <= STH_OP_MAX && jb.stencil.length <= 127) {
-      const int L = jb.stencil.length, nd = jb.stencil.ndim;
-      for (const auto& bv : batch_variants()) {
-        if (bv.dtype != dt || bv.op != jb.op.kind || bv.L != L || bv.ndim != nd) continue;
-        bool eq = true;
-        for (int k = 0; k < L && eq; ++k)
-          for (int d = 0; d < 3 && eq; ++d) {
-            const int o = d < nd ? jb.stencil.offsets[k * nd + d] : 0;
-            eq = bv.offs[3 * k + d] == o;
-          }
-        if (eq) {
-          v = &bv;
-          break;
-        }
-      }
-    }
-    if (!v) {
-      if (pl.fn || pl.halo) serial.push_back(i);
-      continue;
-    }
-    BatchJob bj;
-    std::memset(static_cast<void*>(&bj), 0, sizeof bj);
-    bj.src = pl.p.src;
-    bj.dst = pl.p.dst;
-    bj.bc = pl.p.bc;
-    bj.read_halo = pl.p.read_halo;
-    bj.padval = pl.p.padval;
-    bj.op = pl.p.op;
-    bj.variant = v->variant;
-    bj.tiles_x = static_cast<int32_t>((pl.p.src.n0 + v->tx - 1) / v->tx);
-    bj.tiles_y = static_cast<int32_t>((pl.p.src.n1 + v->ty - 1) / v->ty);
-    bj.tiles_z = static_cast<int32_t>(pl.p.src.n2);
-    groups[dt].push_back(bj);
-  }
-  for (int32_t i : serial)
-    if ((rc = run_plan(plans[i], s))) return rc;
-  for (auto& kv : groups) {
-    const BatchKernelEntry* ke = nullptr;
-    for (const auto& e : batch_kernels())
-      if (e.dtype == kv.first) ke = &e;
-    if (!ke) return fail(STH_EDEVICE, "no batched kernel for dtype %d", kv.first);
-    std::vector<BatchJob>& bj = kv.second;
-    int64_t total = 0;
-    for (auto& j : bj) {
-      j.tile_begin = total;
-      total += static_cast<int64_t>(j.tiles_x) * j.tiles_y * j.tiles_z;
-    }
-    if (total == 0) continue;
-    const size_t bytes = bj.size() * sizeof(BatchJob);
-    std::lock_guard<std::mutex> lk(ctx->mu);
-    BatchSlot& sl = ctx->slots[ctx->next_slot++ % 4];
-    if (sl.used) STH_HIP(hipEventSynchronize(sl.done));  // its last launch has read it
-    if (sl.bytes < bytes) {
-      if (sl.host) STH_HIP(hipHostFree(sl.host));
-      if (sl.dev) STH_HIP(hipFree(sl.dev));
-      sl.host = sl.dev = nullptr;
-      sl.bytes = 0;
-      STH_HIP(hipHostMalloc(&sl.host, bytes, hipHostMallocDefault));
-      STH_HIP(hipMalloc(&sl.dev, bytes));
-      sl.bytes = bytes;
-    }
-    if (!sl.done) STH_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
-    std::memcpy(sl.host, bj.data(), bytes);
-    STH_HIP(hipMemcpyAsync(sl.dev, sl.host, bytes, hipMemcpyHostToDevice, s));
-    BatchParams bp;
-    bp.jobs = static_cast<const BatchJob*>(sl.dev);
-    bp.njobs = static_cast<int32_t>(bj.size());
-    bp.pad_ = 0;
-    bp.total_tiles = total;
-    const int64_t resident = static_cast<int64_t>(ke->occupancy(ke->lds)) * ctx->cus;
-    const unsigned blocks = static_cast<unsigned>(std::min<int64_t>(total, resident));
-    ke->fn(bp, blocks, ke->lds, s);
-    STH_HIP(hipGetLastError());
-    STH_HIP(hipEventRecord(sl.done, s));
-    sl.used = true;
-  }
+  if ((rc = launch_batched(ctx, ptrs.data(), njobs, serial_only, s))) return rc;
   if (!(flags & STH_ASYNC)) STH_HIP(hipStreamSynchronize(s));
   return STH_OK;
   STH_GUARD_END
 }
 
 }  // extern "C"
+
+#include "server.hpp"

@@ -18,7 +18,8 @@ enum TileCfg : int {
   CFG_RT1D = 2,      // 256 x 1 x 1, runtime offsets
   CFG_RT2D = 3,      // 64 x 16 x 1, runtime offsets
   CFG_RT3D = 4,      // 32 x 8 x 4, runtime offsets
-  CFG_LIFE = 5       // SWAR Life kernel (1024 x 4*KY cells per block)
+  CFG_LIFE = 5,      // SWAR Life kernel (1024 x 4*KY cells per block)
+  CFG_STREAM = 6     // streaming kernel: 64*V x 4*KY cells of one plane per block
 };
 
 struct TileDims {
@@ -44,6 +45,8 @@ struct KernelEntry {
   const int8_t* offs;    // static shapes: L x 3 offsets
   LaunchFn fn;
   const char* name;
+  int sv = 0;            // streaming kernels: cells per lane along dim 1
+  int sky = 0;           // streaming kernels: rows per wave along dim 2
 };
 
 std::vector<KernelEntry>& registry();

@@ -1,0 +1,468 @@
+// Serving loop of libstencils_hip (include/stencils_hip.h, sth_server_*):
+// continuous batching of iterated stencil solves with admission by an HBM
+// budget.  Part of api.hip's translation unit (it drives the same plans,
+// tuner and batched launcher).
+//
+// Scheduler thread, one round at a time:
+//   1. retire  -- requests whose result copy has completed on the device get
+//                 a completion record (device timestamps) and free their
+//                 buffers;
+//   2. admit   -- pending requests, in arrival order, whose arrival time has
+//                 passed and whose two buffers fit in the arena (strict FIFO:
+//                 a request that does not fit holds back the ones behind it,
+//                 so large requests are not starved);
+//   3. advance -- K = min(round_steps, fewest remaining steps) steps; each
+//                 step is one persistent launch over every active request
+//                 (requests join and leave only between rounds), buffers
+//                 ping-pong per request; requests that reach their step count
+//                 get their state copied to `result` and an event recorded.
+// Launches are stream-ordered, so a freed buffer is reused only by work
+// enqueued after the copy that last read it.  The host runs at most
+// kBatchSlots launches ahead of the device (the batched launcher's job-table
+// ring waits for the oldest), which bounds the admission delay.
+#pragma once
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <thread>
+
+struct sth_server {
+  struct Req {
+    sth_request r{};
+    std::vector<int32_t> offs;
+    std::vector<unsigned char> weights;
+    uint64_t padval = 0;
+    int64_t cells = 0;
+    size_t pitch = 0;       // bytes per parent row in the arena
+    size_t row_bytes = 0;   // bytes per parent row in the caller's layout
+    int64_t rows = 0;       // parent rows (all dims but the first)
+    size_t bytes_one = 0;   // one buffer
+    size_t off[2] = {0, 0};
+    sth::Plan plan[2];      // plan[c]: buffer c -> buffer 1 - c
+    int64_t done = 0;
+    int cur = 0;
+    double admitted = 0;
+    hipEvent_t fin = nullptr;
+  };
+
+  sth_ctx* ctx = nullptr;
+  hipStream_t stream = nullptr;
+  sth_server_config cfg{};
+  char* arena = nullptr;
+  std::map<size_t, size_t> free_blocks;  // offset -> bytes
+  size_t in_use = 0, peak = 0;
+
+  std::mutex mu;
+  std::condition_variable cv;       // wakes the scheduler (submit / stop)
+  std::condition_variable done_cv;  // wakes pollers
+  std::deque<std::unique_ptr<Req>> pending;
+  std::vector<std::unique_ptr<Req>> active;
+  std::vector<std::unique_ptr<Req>> retiring;
+  std::deque<sth_completion> completed;
+  sth_server_stats stats{};
+  int32_t error = STH_OK;
+  std::string error_msg;
+  bool started = false, stop = false;
+  std::thread thr;
+  std::chrono::steady_clock::time_point t0;
+  hipEvent_t ev0 = nullptr;
+
+  double now() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+
+  bool alloc(size_t bytes, size_t* off) {
+    for (auto it = free_blocks.begin(); it != free_blocks.end(); ++it) {
+      if (it->second < bytes) continue;
+      *off = it->first;
+      const size_t rest = it->second - bytes;
+      const size_t at = it->first + bytes;
+      free_blocks.erase(it);
+      if (rest) free_blocks[at] = rest;
+      in_use += bytes;
+      peak = std::max(peak, in_use);
+      return true;
+    }
+    return false;
+  }
+  void release(size_t off, size_t bytes) {
+    in_use -= bytes;
+    auto it = free_blocks.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != free_blocks.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_blocks.erase(nx);
+    }
+    if (it != free_blocks.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_blocks.erase(it);
+      }
+    }
+  }
+
+  sth_array buffer_desc(const Req& q, int c) const {
+    sth_array a = q.r.state;
+    a.data = arena + q.off[c];
+    const int64_t es = sth::dtype_size(a.dtype);
+    const int64_t h = a.halo;
+    a.stride[0] = 1;
+    a.stride[1] = a.ndim > 1 ? static_cast<int64_t>(q.pitch) / es : 0;
+    a.stride[2] = a.ndim > 2 ? a.stride[1] * (a.size[1] + 2 * h) : 0;
+    return a;
+  }
+
+  int fail_all(int rc) {
+    error = rc;
+    error_msg = sth::g_err;
+    return rc;
+  }
+
+  // --- 2. admission (called with `mu` held)
+  int admit(double t) {
+    int rc = STH_OK;
+    while (!pending.empty() && pending.front()->r.arrival <= t &&
+           (cfg.max_active <= 0 || static_cast<int32_t>(active.size()) < cfg.max_active)) {
+      Req& q = *pending.front();
+      size_t o0 = 0, o1 = 0;
+      if (!alloc(q.bytes_one, &o0)) break;
+      if (!alloc(q.bytes_one, &o1)) {
+        release(o0, q.bytes_one);
+        break;
+      }
+      q.off[0] = o0;
+      q.off[1] = o1;
+      sth_stencil st = q.r.stencil;
+      st.offsets = q.offs.data();
+      st.weights = q.weights.empty() ? nullptr : q.weights.data();
+      const sth_array a = buffer_desc(q, 0), b = buffer_desc(q, 1);
+      for (int c = 0; c < 2 && rc == STH_OK; ++c) {
+        std::vector<sth::Variant> cands;
+        std::string group;
+        rc = sth::make_plan(c ? &b : &a, c ? &a : &b, &st, q.r.boundary, &q.padval, q.r.padding, &q.r.op, 0,
+                            &q.plan[c], &cands, &group);
+        int idx = 0;
+        if (rc == STH_OK) rc = tune_pick(ctx, q.plan[c], cands, group, 0, stream, false, true, &idx);
+        if (rc == STH_OK) sth::apply_variant(q.plan[c], cands[idx]);
+      }
+      if (rc) return rc;
+      hipError_t e = hipMemcpy2DAsync(arena + o0, q.pitch, q.r.state.data, q.row_bytes, q.row_bytes,
+                                      static_cast<size_t>(q.rows), hipMemcpyDefault, stream);
+      if (e != hipSuccess) return sth::fail(STH_EDEVICE, "copy-in: %s", hipGetErrorString(e));
+      e = hipEventCreate(&q.fin);
+      if (e != hipSuccess) return sth::fail(STH_EDEVICE, "hipEventCreate: %s", hipGetErrorString(e));
+      q.admitted = t;
+      active.push_back(std::move(pending.front()));
+      pending.pop_front();
+    }
+    return rc;
+  }
+
+  // --- 1. retirement (called with `mu` held)
+  int retire() {
+    for (size_t i = 0; i < retiring.size();) {
+      Req& q = *retiring[i];
+      const hipError_t e = hipEventQuery(q.fin);
+      if (e == hipErrorNotReady) {
+        ++i;
+        continue;
+      }
+      if (e != hipSuccess) return sth::fail(STH_EDEVICE, "request event: %s", hipGetErrorString(e));
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, ev0, q.fin) != hipSuccess) ms = static_cast<float>(now() * 1e3);
+      sth_completion c{};
+      c.tag = q.r.tag;
+      c.status = STH_OK;
+      c.arrival = q.r.arrival;
+      c.admitted = q.admitted;
+      c.finished = ms * 1e-3;
+      c.steps = q.r.steps;
+      c.cells = q.cells;
+      completed.push_back(c);
+      stats.completed += 1;
+      stats.cell_steps += q.cells * q.r.steps;
+      release(q.off[0], q.bytes_one);
+      release(q.off[1], q.bytes_one);
+      (void)hipEventDestroy(q.fin);
+      retiring.erase(retiring.begin() + static_cast<std::ptrdiff_t>(i));
+      done_cv.notify_all();
+    }
+    return STH_OK;
+  }
+
+  int finish(Req& q) {
+    const int c = q.cur;
+    hipError_t e = hipMemcpy2DAsync(q.r.result, q.row_bytes, arena + q.off[c], q.pitch, q.row_bytes,
+                                    static_cast<size_t>(q.rows), hipMemcpyDefault, stream);
+    if (e != hipSuccess) return sth::fail(STH_EDEVICE, "copy-out: %s", hipGetErrorString(e));
+    e = hipEventRecord(q.fin, stream);
+    if (e != hipSuccess) return sth::fail(STH_EDEVICE, "hipEventRecord: %s", hipGetErrorString(e));
+    return STH_OK;
+  }
+
+  // --- 3. one round (called without `mu`: only the scheduler touches `active`)
+  int advance(int64_t* launches) {
+    int64_t K = cfg.round_steps > 0 ? cfg.round_steps : 8;
+    for (auto& q : active) K = std::min(K, q->r.steps - q->done);
+    const bool serial = (cfg.flags & STH_SERVER_SERIAL) != 0;
+    if (serial) K = active.empty() ? 0 : active.front()->r.steps - active.front()->done;
+    std::vector<const sth::Plan*> plans(active.size());
+    int rc = STH_OK;
+    for (int64_t k = 0; k < K; ++k) {
+      for (size_t i = 0; i < active.size(); ++i) plans[i] = &active[i]->plan[active[i]->cur];
+      if (serial) {
+        for (const sth::Plan* p : plans)
+          if ((rc = sth::run_plan(*p, stream))) return rc;
+      } else if ((rc = launch_batched(ctx, plans.data(), static_cast<int32_t>(plans.size()), false, stream))) {
+        return rc;
+      }
+      for (auto& q : active) {
+        q->cur ^= 1;
+        q->done += 1;
+      }
+      *launches += 1;
+    }
+    return STH_OK;
+  }
+
+  void loop() {
+    (void)hipSetDevice(ctx->device);
+    std::unique_lock<std::mutex> lk(mu);
+    while (true) {
+      int rc = retire();
+      if (rc == STH_OK) rc = admit(now());
+      if (rc) {
+        fail_all(rc);
+        done_cv.notify_all();
+        return;
+      }
+      // requests with nothing left to do (zero steps) finish at once
+      for (size_t i = 0; i < active.size();) {
+        if (active[i]->done >= active[i]->r.steps) {
+          if ((rc = finish(*active[i]))) {
+            fail_all(rc);
+            done_cv.notify_all();
+            return;
+          }
+          retiring.push_back(std::move(active[i]));
+          active.erase(active.begin() + static_cast<std::ptrdiff_t>(i));
+        } else {
+          ++i;
+        }
+      }
+      stats.active = static_cast<int64_t>(active.size());
+      stats.pending = static_cast<int64_t>(pending.size());
+      stats.hbm_in_use = in_use;
+      stats.hbm_peak = peak;
+      if (active.empty()) {
+        if (stop && pending.empty() && retiring.empty()) return;
+        // wait for the next arrival, a submission, or a retiring copy
+        double wait = 1e-3;
+        if (!pending.empty()) wait = std::min(wait, std::max(0.0, pending.front()->r.arrival - now()));
+        if (!retiring.empty()) wait = std::min(wait, 50e-6);
+        cv.wait_for(lk, std::chrono::duration<double>(wait));
+        continue;
+      }
+      lk.unlock();
+      int64_t launches = 0;
+      rc = advance(&launches);
+      lk.lock();
+      stats.launches += launches;
+      stats.rounds += 1;
+      if (rc) {
+        fail_all(rc);
+        done_cv.notify_all();
+        return;
+      }
+    }
+  }
+};
+
+namespace {
+
+int server_validate(const sth_request* q) {
+  if (!q) return sth::fail(STH_EINVAL, "request is NULL");
+  if (!q->state.data || !q->result) return sth::fail(STH_EINVAL, "request state or result is NULL");
+  if (q->steps < 0) return sth::fail(STH_EINVAL, "negative step count");
+  if (q->stencil.length < 1 || !q->stencil.offsets || q->stencil.ndim < 1)
+    return sth::fail(STH_EINVAL, "request stencil has no offsets");
+  if (q->state.ndim < 1 || q->state.ndim > 3 || sth::dtype_size(q->state.dtype) == 0)
+    return sth::fail(STH_EINVAL, "request state descriptor is invalid");
+  if (q->op.kind == STH_OP_MEAN && !sth::is_float(q->state.dtype))
+    return sth::fail(STH_EINVAL, "mapstencil! cannot store a Float64 mean into an integer buffer");
+  return STH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sth_server_create(sth_ctx* ctx, const sth_server_config* cfg, void* stream, sth_server** out) {
+  STH_GUARD_BEGIN
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  if (!cfg || !out) return fail(STH_EINVAL, "config or out is NULL");
+  if (cfg->hbm_budget < 4096) return fail(STH_EINVAL, "hbm_budget of %zu bytes is too small", cfg->hbm_budget);
+  auto srv = std::make_unique<sth_server>();
+  srv->ctx = ctx;
+  srv->stream = static_cast<hipStream_t>(stream);
+  srv->cfg = *cfg;
+  if (cfg->flags & STH_SERVER_SERIAL) srv->cfg.max_active = 1;  // one request at a time
+  const size_t bytes = cfg->hbm_budget & ~static_cast<size_t>(255);
+  hipError_t e = hipMalloc(&srv->arena, bytes);
+  if (e != hipSuccess) return fail(STH_ENOMEM, "arena of %zu bytes: %s", bytes, hipGetErrorString(e));
+  srv->free_blocks[0] = bytes;
+  e = hipEventCreate(&srv->ev0);
+  if (e != hipSuccess) {
+    (void)hipFree(srv->arena);
+    return fail(STH_EDEVICE, "hipEventCreate: %s", hipGetErrorString(e));
+  }
+  *out = srv.release();
+  return STH_OK;
+  STH_GUARD_END
+}
+
+int sth_server_submit(sth_server* srv, const sth_request* q) {
+  STH_GUARD_BEGIN
+  if (!srv) return fail(STH_EINVAL, "server is NULL");
+  int rc = server_validate(q);
+  if (rc) return rc;
+  auto r = std::make_unique<sth_server::Req>();
+  r->r = *q;
+  const int nd = q->stencil.ndim;
+  r->offs.assign(q->stencil.offsets, q->stencil.offsets + static_cast<size_t>(q->stencil.length) * nd);
+  if (q->stencil.weights) {
+    const int ws = dtype_size(q->stencil.weights_dtype);
+    if (ws == 0) return fail(STH_EINVAL, "unknown weights dtype %d", q->stencil.weights_dtype);
+    const auto* w = static_cast<const unsigned char*>(q->stencil.weights);
+    r->weights.assign(w, w + static_cast<size_t>(q->stencil.length) * ws);
+  }
+  const int es = dtype_size(q->state.dtype);
+  if (q->padval) std::memcpy(&r->padval, q->padval, static_cast<size_t>(es));
+  const int64_t h = q->state.halo;
+  const int ndim = q->state.ndim;
+  int64_t cells = 1, rows = 1;
+  for (int d = 0; d < ndim; ++d) {
+    if (q->state.size[d] < 1) return fail(STH_EINVAL, "request state has an empty dim");
+    cells *= q->state.size[d];
+    if (d > 0) rows *= q->state.size[d] + 2 * h;
+  }
+  r->cells = cells;
+  r->rows = rows;
+  r->row_bytes = static_cast<size_t>((q->state.size[0] + 2 * h) * es);
+  if (ndim > 1 && q->state.stride[1] != q->state.size[0] + 2 * h)
+    return fail(STH_EINVAL, "request state must be dense (stride[1] = size[0] + 2 halo)");
+  r->pitch = (r->row_bytes + 63) & ~static_cast<size_t>(63);
+  r->bytes_one = (r->pitch * static_cast<size_t>(rows) + 255) & ~static_cast<size_t>(255);
+  // validate the call once, against stand-in buffers of the arena layout
+  {
+    sth_array a = srv->buffer_desc(*r, 0), b = a;
+    a.data = reinterpret_cast<void*>(uintptr_t(1) << 20);
+    b.data = reinterpret_cast<void*>((uintptr_t(1) << 20) + 2 * r->bytes_one + 4096);
+    sth_stencil st = q->stencil;
+    st.offsets = r->offs.data();
+    st.weights = r->weights.empty() ? nullptr : r->weights.data();
+    Plan tmp;
+    if ((rc = make_plan(&a, &b, &st, q->boundary, &r->padval, q->padding, &q->op, 0, &tmp))) return rc;
+  }
+  std::lock_guard<std::mutex> lk(srv->mu);
+  if (2 * r->bytes_one > (srv->cfg.hbm_budget & ~static_cast<size_t>(255)))
+    return fail(STH_ENOMEM, "request needs %zu bytes; the server budget is %zu", 2 * r->bytes_one,
+                srv->cfg.hbm_budget);
+  // keep `pending` in arrival order (stable for equal arrivals)
+  auto it = srv->pending.end();
+  while (it != srv->pending.begin() && (*std::prev(it))->r.arrival > r->r.arrival) --it;
+  srv->pending.insert(it, std::move(r));
+  srv->stats.submitted += 1;
+  srv->cv.notify_all();
+  return STH_OK;
+  STH_GUARD_END
+}
+
+int sth_server_start(sth_server* srv) {
+  STH_GUARD_BEGIN
+  if (!srv) return fail(STH_EINVAL, "server is NULL");
+  int rc = set_device(srv->ctx);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(srv->mu);
+  if (srv->started) return fail(STH_EINVAL, "server already started");
+  STH_HIP(hipStreamSynchronize(srv->stream));
+  STH_HIP(hipEventRecord(srv->ev0, srv->stream));
+  STH_HIP(hipEventSynchronize(srv->ev0));
+  srv->t0 = std::chrono::steady_clock::now();
+  srv->started = true;
+  srv->thr = std::thread([srv] { srv->loop(); });
+  return STH_OK;
+  STH_GUARD_END
+}
+
+int sth_server_poll(sth_server* srv, sth_completion* out, int32_t max, int32_t* n, int32_t wait_ms) {
+  STH_GUARD_BEGIN
+  if (!srv || !n || (max > 0 && !out)) return fail(STH_EINVAL, "bad poll arguments");
+  std::unique_lock<std::mutex> lk(srv->mu);
+  if (srv->completed.empty() && wait_ms > 0 && srv->error == STH_OK)
+    srv->done_cv.wait_for(lk, std::chrono::milliseconds(wait_ms),
+                          [srv] { return !srv->completed.empty() || srv->error != STH_OK; });
+  int32_t k = 0;
+  while (k < max && !srv->completed.empty()) {
+    out[k++] = srv->completed.front();
+    srv->completed.pop_front();
+  }
+  *n = k;
+  if (srv->error) return fail(srv->error, "server stopped: %s", srv->error_msg.c_str());
+  return STH_OK;
+  STH_GUARD_END
+}
+
+int sth_server_drain(sth_server* srv, double timeout_s) {
+  STH_GUARD_BEGIN
+  if (!srv) return fail(STH_EINVAL, "server is NULL");
+  std::unique_lock<std::mutex> lk(srv->mu);
+  if (!srv->started) return fail(STH_EINVAL, "server not started");
+  const bool ok = srv->done_cv.wait_for(lk, std::chrono::duration<double>(timeout_s), [srv] {
+    return srv->error != STH_OK || srv->stats.completed == srv->stats.submitted;
+  });
+  if (srv->error) return fail(srv->error, "server stopped: %s", srv->error_msg.c_str());
+  if (!ok) return fail(STH_EDEVICE, "drain timed out with %lld of %lld requests done",
+                       static_cast<long long>(srv->stats.completed), static_cast<long long>(srv->stats.submitted));
+  return STH_OK;
+  STH_GUARD_END
+}
+
+int sth_server_stats_get(sth_server* srv, sth_server_stats* out) {
+  STH_GUARD_BEGIN
+  if (!srv || !out) return fail(STH_EINVAL, "bad arguments");
+  std::lock_guard<std::mutex> lk(srv->mu);
+  *out = srv->stats;
+  out->hbm_in_use = srv->in_use;
+  out->hbm_peak = srv->peak;
+  out->started = srv->started ? srv->now() : 0.0;
+  return STH_OK;
+  STH_GUARD_END
+}
+
+int sth_server_destroy(sth_server* srv) {
+  STH_GUARD_BEGIN
+  if (!srv) return STH_OK;
+  (void)hipSetDevice(srv->ctx->device);
+  {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    srv->stop = true;
+    srv->cv.notify_all();
+  }
+  if (srv->thr.joinable()) srv->thr.join();
+  (void)hipStreamSynchronize(srv->stream);
+  for (auto* v : {&srv->active, &srv->retiring})
+    for (auto& q : *v)
+      if (q->fin) (void)hipEventDestroy(q->fin);
+  if (srv->ev0) (void)hipEventDestroy(srv->ev0);
+  if (srv->arena) (void)hipFree(srv->arena);
+  delete srv;
+  return STH_OK;
+  STH_GUARD_END
+}
+
+}  // extern "C"

@@ -106,71 +106,87 @@ __device__ __forceinline__ A fold_max(A a, A b) {
   else return b > a ? b : a;
 }
 
+// One step of the left fold over the stencil values, in offset order: `k` is
+// the offset's position, `rv` its staged value.  Shared by every kernel so
+// that all paths compute each cell with the same expressions.
+template <typename S, int OPK, class P>
+__device__ __forceinline__ void fold_step(typename AccOf<S, OPK>::type& acc, uint32_t& cnt, int k,
+                                          typename Staged<S>::type rv, const P& p) {
+  using A = typename AccOf<S, OPK>::type;
+  if constexpr (OPK == OP_LIFE) {
+    cnt += nonzero(rv) ? 1u : 0u;
+  } else {
+    A v;
+    if constexpr (kStorageMode<S>) v = rv;
+    else v = widen<A, S>(rv);
+    if constexpr (OPK == OP_SUM || OPK == OP_MEAN || OPK == OP_LAP) {
+      acc = k == 0 ? v : acc + v;
+    } else if constexpr (OPK == OP_MIN) {
+      acc = k == 0 ? v : fold_min(acc, v);
+    } else if constexpr (OPK == OP_MAX) {
+      acc = k == 0 ? v : fold_max(acc, v);
+    } else if constexpr (OPK == OP_KP) {
+      if constexpr (kStorageMode<S>) {
+        // storage precision mode: f32 fused multiply-add, offset order
+        acc = __builtin_fmaf(v, weight<A>(p, k), acc);
+      } else {
+        // native types: separate multiply and add, as the reference
+        const A pr = v * weight<A>(p, k);
+        acc = acc + pr;
+      }
+    }
+  }
+}
+
+// Finish the fold: mean divides by L, the Laplacian and Life rules combine
+// with the centre value `cv`.
+template <typename S, int OPK, class P>
+__device__ __forceinline__ typename AccOf<S, OPK>::type fold_finish(typename AccOf<S, OPK>::type acc,
+                                                                    uint32_t cnt,
+                                                                    typename Staged<S>::type cv,
+                                                                    int L, const P& p) {
+  using A = typename AccOf<S, OPK>::type;
+  if constexpr (OPK == OP_MEAN) {
+    acc = acc / static_cast<A>(L);
+  } else if constexpr (OPK == OP_LAP) {
+    A c;
+    if constexpr (kStorageMode<S>) c = cv;
+    else c = widen<A, S>(cv);
+    const A t = static_cast<A>(L) * c;
+    const A d = acc - t;
+    const A a = static_cast<A>(p.op.alpha) * d;
+    acc = c + a;
+  } else if constexpr (OPK == OP_LIFE) {
+    const bool alive = nonzero(cv);
+    const uint64_t mask = alive ? p.op.survive : p.op.birth;
+    acc = cnt < 64u ? static_cast<A>((mask >> cnt) & 1u) : A(0);
+  }
+  return acc;
+}
+
 // One output value from the staged box; `c` is the LDS index of the centre.
 template <typename S, int OPK, class Sh, int W, int HW, class P>
 __device__ __forceinline__ typename AccOf<S, OPK>::type reduce_cell(
     const typename Staged<S>::type* __restrict__ tile, int c, const P& p,
     int wdyn, int hwdyn) {
   using A = typename AccOf<S, OPK>::type;
-  using Raw = typename Staged<S>::type;
   const int L = Sh::kStatic ? Sh::L : p.L;
   A acc = A(0);
   uint32_t cnt = 0;
-  auto step = [&](int k, int lin) {
-    const Raw rv = tile[c + lin];
-    if constexpr (OPK == OP_LIFE) {
-      cnt += nonzero(rv) ? 1u : 0u;
-    } else {
-      A v;
-      if constexpr (kStorageMode<S>) v = rv;
-      else v = widen<A, S>(rv);
-      if constexpr (OPK == OP_SUM || OPK == OP_MEAN || OPK == OP_LAP) {
-        acc = k == 0 ? v : acc + v;
-      } else if constexpr (OPK == OP_MIN) {
-        acc = k == 0 ? v : fold_min(acc, v);
-      } else if constexpr (OPK == OP_MAX) {
-        acc = k == 0 ? v : fold_max(acc, v);
-      } else if constexpr (OPK == OP_KP) {
-        if constexpr (kStorageMode<S>) {
-          // storage precision mode: f32 fused multiply-add, offset order
-          acc = __builtin_fmaf(v, weight<A>(p, k), acc);
-        } else {
-          // native types: separate multiply and add, as the reference
-          const A pr = v * weight<A>(p, k);
-          acc = acc + pr;
-        }
-      }
-    }
-  };
   if constexpr (Sh::kStatic) {
 #pragma unroll
     for (int k = 0; k < Sh::L; ++k)
-      step(k, Sh::off[k][2] * HW + Sh::off[k][1] * W + Sh::off[k][0]);
+      fold_step<S, OPK>(acc, cnt, k, tile[c + Sh::off[k][2] * HW + Sh::off[k][1] * W + Sh::off[k][0]], p);
   } else {
     for (int k = 0; k < L; ++k) {
       const int o = p.offs[k];
       const int dx = static_cast<int8_t>(o & 0xff);
       const int dy = static_cast<int8_t>((o >> 8) & 0xff);
       const int dz = static_cast<int8_t>((o >> 16) & 0xff);
-      step(k, dz * hwdyn + dy * wdyn + dx);
+      fold_step<S, OPK>(acc, cnt, k, tile[c + dz * hwdyn + dy * wdyn + dx], p);
     }
   }
-  if constexpr (OPK == OP_MEAN) {
-    acc = acc / static_cast<A>(L);
-  } else if constexpr (OPK == OP_LAP) {
-    A cv;
-    if constexpr (kStorageMode<S>) cv = tile[c];
-    else cv = widen<A, S>(tile[c]);
-    const A t = static_cast<A>(L) * cv;
-    const A d = acc - t;
-    const A a = static_cast<A>(p.op.alpha) * d;
-    acc = cv + a;
-  } else if constexpr (OPK == OP_LIFE) {
-    const bool alive = nonzero(tile[c]);
-    const uint64_t mask = alive ? p.op.survive : p.op.birth;
-    acc = cnt < 64u ? static_cast<A>((mask >> cnt) & 1u) : A(0);
-  }
-  return acc;
+  return fold_finish<S, OPK>(acc, cnt, tile[c], L, p);
 }
 
 template <typename Raw>

@@ -11,6 +11,7 @@ from .array import (StencilArray, SwitchingStencilArray, colmajor_empty, dest,
 from .boundary import Conditional, Halo, Reflect, Remove, Use, Wrap, padval
 from .gather import (BatchPlan, gatherstencil, gatherstencil_, mapstencil, mapstencil_,
                      mapstencil_batch, mapstencil_steps, result_dtype)
+from .server import Completion, Server, serve_latency_summary
 from .ops import (Laplacian, LifeRule, Unsupported, kernelproduct, maximum, mean,
                   minimum, sum)
 from .stencil import (AngledCross, Annulus, BackSlash, Cardinal, Circle, Cross,

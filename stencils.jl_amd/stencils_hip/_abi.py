@@ -43,6 +43,7 @@ ASYNC = 1
 GRAPH = 2
 GENERIC = 4
 BATCH_SERIAL = 8
+NO_TUNE = 16
 
 
 class SthArray(ctypes.Structure):
@@ -128,3 +129,35 @@ def make_array(data_ptr: int, dtype: int, shape, strides_elems, halo: int) -> St
         a.stride[d] = int(strides_elems[d]) if d < len(shape) else 0
     a.halo = int(halo)
     return a
+
+
+# serving loop (sth_server_*)
+SERVER_SERIAL = 1
+
+
+class SthServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_size_t), ("max_active", ctypes.c_int32),
+                ("round_steps", ctypes.c_int32), ("flags", ctypes.c_uint32),
+                ("reserved", ctypes.c_int32)]
+
+
+class SthRequest(ctypes.Structure):
+    _fields_ = [("state", SthArray), ("stencil", SthStencil), ("op", SthOp),
+                ("padval", ctypes.c_void_p), ("boundary", ctypes.c_int32),
+                ("padding", ctypes.c_int32), ("steps", ctypes.c_int64),
+                ("result", ctypes.c_void_p), ("arrival", ctypes.c_double),
+                ("tag", ctypes.c_uint64)]
+
+
+class SthCompletion(ctypes.Structure):
+    _fields_ = [("tag", ctypes.c_uint64), ("status", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("arrival", ctypes.c_double), ("admitted", ctypes.c_double),
+                ("finished", ctypes.c_double), ("steps", ctypes.c_int64), ("cells", ctypes.c_int64)]
+
+
+class SthServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64),
+                ("active", ctypes.c_int64), ("pending", ctypes.c_int64),
+                ("rounds", ctypes.c_int64), ("launches", ctypes.c_int64),
+                ("cell_steps", ctypes.c_int64), ("hbm_in_use", ctypes.c_size_t),
+                ("hbm_peak", ctypes.c_size_t), ("started", ctypes.c_double)]

@@ -40,6 +40,23 @@ EXPORTS = {
                                    ctypes.POINTER(ctypes.c_int32), ctypes.c_void_p]),
     "sth_map_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(_abi.SthJob),
                                      ctypes.c_uint32, ctypes.c_void_p]),
+    "sth_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_abi.SthArray),
+                                    ctypes.POINTER(_abi.SthArray), ctypes.POINTER(_abi.SthStencil),
+                                    ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32,
+                                    ctypes.POINTER(_abi.SthOp), ctypes.c_void_p, ctypes.c_char_p,
+                                    ctypes.c_size_t]),
+    "sth_tuning_table": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t,
+                                        ctypes.POINTER(ctypes.c_size_t)]),
+    "sth_set_variant": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    "sth_server_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_abi.SthServerConfig),
+                                         ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "sth_server_submit": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_abi.SthRequest)]),
+    "sth_server_start": (ctypes.c_int, [ctypes.c_void_p]),
+    "sth_server_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_abi.SthCompletion), ctypes.c_int32,
+                                       ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]),
+    "sth_server_drain": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    "sth_server_stats_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_abi.SthServerStats)]),
+    "sth_server_destroy": (ctypes.c_int, [ctypes.c_void_p]),
 }
 
 
@@ -92,6 +109,21 @@ def check(status: int):
     if status == _abi.STH_EINVAL:
         raise ValueError(msg)
     raise StencilsHipError(status, msg)
+
+
+def tuning_table(device: int = 0) -> dict:
+    """The autotuner's decisions: {family|bucket: variant}."""
+    L = lib()
+    need = ctypes.c_size_t(0)
+    check(L.sth_tuning_table(context(device), None, 0, ctypes.byref(need)))
+    buf = ctypes.create_string_buffer(max(need.value, 1))
+    check(L.sth_tuning_table(context(device), buf, need.value, None))
+    return dict(line.split("\t", 1) for line in buf.value.decode().splitlines() if "\t" in line)
+
+
+def set_variant(pattern: str = "", device: int = 0):
+    """Force the first kernel variant whose name contains `pattern` ("" = tuner)."""
+    check(lib().sth_set_variant(context(device), pattern.encode()))
 
 
 def context(device: int = 0):

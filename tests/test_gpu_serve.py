@@ -1,0 +1,148 @@
+"""Serving loop (sth_server_*): every request's result is the same alone and
+in the mix -- bit for bit against the same steps run by themselves
+(sth_iterate) and against the oracle -- in batched and serial modes, under a
+budget that makes requests wait for memory."""
+import numpy as np
+import pytest
+
+import stencils_hip as sh
+
+pytestmark = pytest.mark.gpu
+
+
+def _mixed_requests(seed, n):
+    """Mixed 3-D heat (f32, bf16), 2-D mean and Life requests."""
+    import torch
+    rng = np.random.default_rng(seed)
+    dev = torch.device("cuda")
+    out = []
+    for i in range(n):
+        kind = i % 5
+        steps = int(rng.integers(0 if i == 3 else 1, 40))
+        if kind in (0, 1):
+            s = int(rng.integers(6, 40))
+            shape = (s, int(rng.integers(6, 40)), int(rng.integers(6, 30)))
+            tdt = torch.float32 if kind == 0 else torch.bfloat16
+            A = sh.colmajor_empty(shape, tdt, dev)
+            A.copy_(torch.rand(shape, device=dev).to(tdt))
+            S = sh.SwitchingStencilArray(A, sh.VonNeumann(1, 3), boundary=sh.Wrap())
+            f = sh.Laplacian(0.15)
+        elif kind == 2:
+            shape = (int(rng.integers(8, 90)), int(rng.integers(8, 60)))
+            A = sh.colmajor_empty(shape, torch.float64, dev)
+            A.copy_(torch.rand(shape, dtype=torch.float64, device=dev))
+            S = sh.SwitchingStencilArray(A, sh.Window(1), boundary=sh.Reflect())
+            f = sh.mean
+        elif kind == 3:
+            shape = (int(rng.integers(10, 30)),) * 3
+            A = sh.colmajor_empty(shape, torch.float32, dev)
+            A.copy_(torch.rand(shape, device=dev))
+            S = sh.SwitchingStencilArray(A, sh.VonNeumann(1, 3), boundary=sh.Remove(np.float32(0)),
+                                         padding=sh.Halo())
+            f = sh.Laplacian(0.15)
+        else:
+            shape = (64 * int(rng.integers(1, 4)), int(rng.integers(8, 50)))
+            A = (rng.random(shape) < 0.35).astype(np.uint8)
+            S = sh.SwitchingStencilArray(A, sh.Moore(1), boundary=sh.Wrap(), padding=sh.Halo())
+            f = sh.LifeRule()
+        out.append((f, S, steps, 0.002 * i))
+    return out
+
+
+def _alone(f, S, steps):
+    """The same request run by itself: `steps` mapstencil! on copies."""
+    import torch
+    from stencils_hip.array import colmajor_copy, is_device
+    src = colmajor_copy(S.source, torch.cuda.current_device())
+    dst = colmajor_copy(S.dest, torch.cuda.current_device())
+    T = sh.SwitchingStencilArray(stencil=S.stencil, boundary=S.boundary, padding=S.padding,
+                                 _buffers=(src, dst))
+    T = sh.mapstencil_steps(f, T, steps, graph=False)
+    return T.source
+
+
+def _serve(reqs, **kw):
+    srv = sh.Server(**kw)
+    tags = {}
+    for f, S, steps, arr in reqs:
+        tags[srv.submit(f, S, steps, arrival=arr)] = (f, S, steps)
+    srv.start()
+    srv.drain(timeout=120)
+    done = srv.completions()
+    stats = srv.stats()
+    srv.close()
+    return done, tags, stats
+
+
+def _interior(S, x):
+    h = S.halo()
+    return x[tuple(slice(h, h + n) for n in S.size())]
+
+
+@pytest.mark.parametrize("serial", [False, True])
+def test_served_equals_alone(gpu, serial):
+    reqs = _mixed_requests(1, 15)
+    done, tags, stats = _serve(reqs, hbm_budget=1 << 28, serial=serial, round_steps=4)
+    assert len(done) == len(reqs) and stats["completed"] == len(reqs)
+    for c in done:
+        f, S, steps = tags[c.tag]
+        import torch
+        ref = _alone(f, S, steps)
+        a = _interior(S, c.result)
+        b = _interior(S, ref)
+        assert a.dtype == b.dtype and torch.equal(a, b), (S, steps)
+        assert c.finished >= c.admitted >= c.arrival - 1e-9
+
+
+def test_served_matches_oracle(gpu, oracle):
+    from stencils_hip.array import to_numpy
+    reqs = [r for r in _mixed_requests(2, 10) if r[1].sth_dtype() != sh._abi.BF16]
+    done, tags, _ = _serve(reqs, hbm_budget=1 << 28, round_steps=3)
+    for c in done:
+        f, S, steps = tags[c.tag]
+        H = sh.SwitchingStencilArray(stencil=S.stencil, boundary=S.boundary, padding=S.padding,
+                                     _buffers=(np.array(to_numpy(S.source), order="F"),
+                                               np.array(to_numpy(S.dest), order="F")))
+        H = oracle.mapstencil_steps(f, H, steps)
+        got = _interior(S, c.result).cpu().numpy()
+        exp = _interior(S, H.source)
+        assert np.array_equal(got, exp), (S, steps)
+
+
+def test_budget_admission(gpu):
+    """A budget that holds about two requests: they queue for memory, all
+    finish, the arena never exceeds the budget and results stay exact."""
+    import torch
+    dev = torch.device("cuda")
+    reqs = []
+    for i in range(6):
+        A = sh.colmajor_empty((48, 48, 48), torch.float32, dev)
+        A.copy_(torch.rand((48, 48, 48), device=dev))
+        reqs.append((sh.Laplacian(0.15), sh.SwitchingStencilArray(A, sh.VonNeumann(1, 3), boundary=sh.Wrap()),
+                     25 + 5 * i, 0.0))
+    one = 48 * 48 * 48 * 4
+    budget = 5 * one  # two requests of two buffers each
+    done, tags, stats = _serve(reqs, hbm_budget=budget, round_steps=4)
+    assert len(done) == 6
+    assert stats["hbm_peak"] <= budget
+    assert stats["hbm_peak"] >= 2 * one
+    admitted = sorted(c.admitted for c in done)
+    assert admitted[-1] > admitted[0]  # later requests waited for memory
+    for c in done:
+        f, S, steps = tags[c.tag]
+        assert torch.equal(c.result, _alone(f, S, steps))
+
+
+def test_request_errors(gpu):
+    import torch
+    dev = torch.device("cuda")
+    srv = sh.Server(hbm_budget=1 << 20)
+    A = sh.colmajor_empty((64, 64, 64), torch.float32, dev)
+    S = sh.SwitchingStencilArray(A, sh.VonNeumann(1, 3), boundary=sh.Wrap())
+    with pytest.raises(sh.StencilsHipError):   # two 1 MiB buffers exceed the budget
+        srv.submit(sh.Laplacian(0.15), S, 3)
+    B = sh.colmajor_empty((4, 4, 4), torch.float32, dev)
+    T = sh.SwitchingStencilArray(B, sh.VonNeumann(1, 3), boundary=sh.Wrap())
+    with pytest.raises(ValueError):
+        srv.submit(sh.Laplacian(0.15), T, -1)
+    srv.close()

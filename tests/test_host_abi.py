@@ -48,6 +48,15 @@ int main(void) {
          offsetof(sth_op, birth_mask), offsetof(sth_op, survive_mask));
   printf("%zu %zu %zu %zu %zu\n", sizeof(sth_job), offsetof(sth_job, stencil),
          offsetof(sth_job, op), offsetof(sth_job, padval), offsetof(sth_job, padding));
+  printf("%zu %zu %zu %zu\n", sizeof(sth_server_config), offsetof(sth_server_config, max_active),
+         offsetof(sth_server_config, round_steps), offsetof(sth_server_config, flags));
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(sth_request), offsetof(sth_request, stencil),
+         offsetof(sth_request, op), offsetof(sth_request, steps), offsetof(sth_request, arrival),
+         offsetof(sth_request, tag));
+  printf("%zu %zu %zu %zu\n", sizeof(sth_completion), offsetof(sth_completion, arrival),
+         offsetof(sth_completion, finished), offsetof(sth_completion, cells));
+  printf("%zu %zu %zu\n", sizeof(sth_server_stats), offsetof(sth_server_stats, hbm_in_use),
+         offsetof(sth_server_stats, started));
   return 0;
 }''')
     exe = tmp_path / "layout"
@@ -59,6 +68,15 @@ int main(void) {
     assert out[2] == [ctypes.sizeof(O), O.alpha.offset, O.birth_mask.offset, O.survive_mask.offset]
     J = _abi.SthJob
     assert out[3] == [ctypes.sizeof(J), J.stencil.offset, J.op.offset, J.padval.offset, J.padding.offset]
+    C = _abi.SthServerConfig
+    assert out[4] == [ctypes.sizeof(C), C.max_active.offset, C.round_steps.offset, C.flags.offset]
+    Q = _abi.SthRequest
+    assert out[5] == [ctypes.sizeof(Q), Q.stencil.offset, Q.op.offset, Q.steps.offset, Q.arrival.offset,
+                      Q.tag.offset]
+    M = _abi.SthCompletion
+    assert out[6] == [ctypes.sizeof(M), M.arrival.offset, M.finished.offset, M.cells.offset]
+    T = _abi.SthServerStats
+    assert out[7] == [ctypes.sizeof(T), T.hbm_in_use.offset, T.started.offset]
 
 
 def test_header_enums_match_python():
