@@ -270,6 +270,82 @@ def bench_batched(sh, torch):
             "speedup_vs_c_loop": wall_s / wall_b, "speedup_vs_python_loop": wall_p / wall_b}
 
 
+def served_requests(sh, torch, n: int, tdt, seed: int = 5, rate: float = 200.0):
+    """Poisson stream (rate/s) of 3-D heat solves: Float32 cube, side uniform
+    in {32..256}, steps uniform in {10..1000}, uniform[0,1) state,
+    VonNeumann(1,3) Laplacian(0.15), Wrap, Conditional (seed 5)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    dev = torch.device("cuda", torch.cuda.current_device())
+    t = 0.0
+    out = []
+    for _ in range(n):
+        t += float(rng.exponential(1.0 / rate))
+        side = int(rng.integers(32, 257))
+        steps = int(rng.integers(10, 1001))
+        g = torch.Generator(device=dev).manual_seed(int(rng.integers(1 << 31)))
+        A = sh.colmajor_empty((side,) * 3, tdt, dev)
+        A.copy_(torch.rand((side,) * 3, generator=g, device=dev).to(tdt))
+        out.append((sh.SwitchingStencilArray(A, sh.VonNeumann(1, 3), boundary=sh.Wrap()), steps, t))
+    return out
+
+
+def bench_served(sh, torch, n: int = 120, budget: int = 2 << 30):
+    """Config 5: the request stream through the continuous-batching server
+    (FP32 and BF16 storage) and through the one-request-at-a-time path with
+    one plain launch per step; latency p50/p99, throughput, and each
+    result's error against the same request run in Float64."""
+    f = sh.Laplacian(0.15)
+    res = {"workload": f"Poisson 200 req/s x {n} requests, 3-D heat VonNeumann(1,3) Laplacian(0.15), "
+                       f"Wrap, cube side U{{32..256}}, steps U{{10..1000}}; HBM budget {budget >> 20} MiB"}
+
+    def run(reqs, serial):
+        srv = sh.Server(hbm_budget=budget, round_steps=8, serial=serial)
+        tags = {}
+        for S, steps, arr in reqs:
+            tags[srv.submit(f, S, steps, arrival=arr)] = (S, steps)
+        torch.cuda.synchronize()
+        srv.start()
+        srv.drain(timeout=600)
+        done = srv.completions()
+        st = srv.stats()
+        srv.close()
+        summ = sh.serve_latency_summary(done)
+        summ.update({"hbm_peak_mib": st["hbm_peak"] / 2**20, "rounds": st["rounds"],
+                     "launches": st["launches"], "gbs": summ["cell_steps_per_s"] * 8 / 1e9})
+        return done, tags, summ
+
+    for name, tdt in (("f32", torch.float32), ("bf16", torch.bfloat16)):
+        reqs = served_requests(sh, torch, n, tdt)
+        done, tags, summ = run(reqs, False)
+        if name == "bf16":
+            summ["gbs"] = summ["cell_steps_per_s"] * 4 / 1e9
+        # FP64 reference of every request: the same steps from the same initial values
+        errs = []
+        for c in done:
+            S, steps = tags[c.tag]
+            a = sh.colmajor_empty(S.size(), torch.float64, S.source.device)
+            a.copy_(S.source.to(torch.float64))
+            b = sh.colmajor_empty(S.size(), torch.float64, S.source.device)
+            R = sh.mapstencil_steps(f, sh.SwitchingStencilArray(
+                stencil=S.stencil, boundary=S.boundary, padding=S.padding, _buffers=(a, b)), steps)
+            d = (c.result.to(torch.float64) - R.source).abs()
+            errs.append((float(d.max()), float((d / R.source.abs().clamp_min(1e-300)).max())))
+            del a, b, R, d
+        summ["max_abs_err_vs_fp64"] = max(e[0] for e in errs)
+        summ["max_rel_err_vs_fp64"] = max(e[1] for e in errs)
+        summ["median_abs_err_vs_fp64"] = float(np.median([e[0] for e in errs]))
+        res[name] = summ
+        if name == "f32":
+            _, _, ser = run(reqs, True)
+            res["f32_serial_plain_launches"] = ser
+            res["served_vs_serial"] = {"throughput": summ["cell_steps_per_s"] / ser["cell_steps_per_s"],
+                                       "p50_latency": ser["p50_s"] / summ["p50_s"],
+                                       "p99_latency": ser["p99_s"] / summ["p99_s"]}
+        del reqs, done, tags
+        torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -301,7 +377,10 @@ def main():
     S, _, _ = time_steps(sh, torch, S, rule, args.warmup, graph=True)
     S, wall, ev = time_steps(sh, torch, S, rule, args.steps, graph=True)
     bytes_per_step = 2 * cells  # 1 B read + 1 B written per interior cell
-    S, launch_us = kernel_launch_us(sh, torch, S, rule)
+    # average duration of one step's kernel: graph-replayed steps between
+    # events on the launch stream (no per-launch host gaps)
+    launch_us = ev / args.steps * 1e6
+    S, plain_launch_us = kernel_launch_us(sh, torch, S, rule)
     achieved = bytes_per_step / (launch_us * 1e-6) / 1e9
 
     # measured stream-copy bandwidth on the same bytes (one read + one write)
@@ -342,7 +421,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": profiled_traffic("life_kernel_gol_8192.md"),
                      "kernel": "life_kernel<KY=4, B3/S23>", "bytes_per_launch": bytes_per_step,
-                     "avg_launch_us": launch_us, "graph_step_us": ev / args.steps * 1e6,
+                     "avg_launch_us": launch_us, "event_bracketed_launch_us": plain_launch_us,
                      "stream_copy_gbs": copy_gbs,
                      "frac_of_copy": achieved / copy_gbs},
         "plain_per_step": {"value": cells * args.steps / wall_plain, "unit": "cells/s",
@@ -354,7 +433,8 @@ def main():
     if not args.no_configs:
         out["configs"] = {"mean_blur_f64": bench_mean_blur(sh, torch, not args.no_cpu_baseline),
                           "gaussian_7x7": bench_gaussian(sh, torch),
-                          "batched_uneven": bench_batched(sh, torch)}
+                          "batched_uneven": bench_batched(sh, torch),
+                          "served_heat_3d": bench_served(sh, torch)}
     print(json.dumps(out), flush=True)
 
 

@@ -1,15 +1,17 @@
 // Batched gather: many independent mapstencil calls of mixed sizes, shapes
-// and reductions in one persistent launch.
+// and reductions in one launch per (element type, stencil shape).
 //
 // The host flattens the tiles of every job into one index space (job j owns
-// tiles [tile_begin_j, tile_begin_{j+1})) and launches as many workgroups as
-// fit on the device at once.  Each workgroup pulls tiles from a device-side
-// counter -- one atomic per tile, issued for the next tile before the
-// current one is processed -- until the counter passes the last tile, so
-// cheap and expensive jobs balance across the device without a host loop.
-// A tile runs the streaming body of its job's (shape, op) variant, selected
-// by a switch generated per element type.  Results are bit-identical to one
-// sth_map call per job (same fold expressions per cell).
+// tiles [tile_begin_j, tile_begin_{j+1})) and launches one workgroup per
+// strip of `tiles_per_block` consecutive tiles; the hardware dispatcher hands
+// strips to CUs as they free up, so cheap and expensive jobs balance without
+// atomics or barriers (a single device-wide atomic tile counter measured much
+// slower: same-address atomics serialise in L2).  A workgroup finds its first
+// job by binary search over the job table (uniform, scalar loads) and walks
+// forward.  Each tile runs the streaming body of its job's (shape, op)
+// variant, selected by a switch generated per element type and class.
+// Results are bit-identical to one sth_map call per job (same fold
+// expressions per cell).
 #pragma once
 
 #include "stream_kernel.hpp"
@@ -30,9 +32,8 @@ struct BatchJob {
 
 struct BatchParams {
   const BatchJob* jobs;
-  unsigned int* queue;  // tile counter; zero when the launch starts
   int32_t njobs;
-  int32_t pad_;
+  int32_t tiles_per_block;
   int64_t total_tiles;
 };
 
@@ -42,32 +43,33 @@ struct BatchParams {
 // sequences that reuse the return-address registers, so the kernel must not
 // call one.)
 template <typename S, class Dispatch>
-__global__ __launch_bounds__(256) void batch_kernel(BatchParams bp) {
-  extern __shared__ __align__(16) unsigned char smem_dyn[];
-  unsigned int& slot = *reinterpret_cast<unsigned int*>(smem_dyn);  // the block's next tile
+__global__ __launch_bounds__(256) void batch_kernel(BatchParams bp, const BatchJob* __restrict__ jobs) {
+  // `jobs` is read-only and aliases nothing the kernel writes, so its fields
+  // stay in scalar registers instead of being reloaded after every store.
   const int lane = static_cast<int>(threadIdx.x) & 63;
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
-  if (threadIdx.x == 0) slot = atomicAdd(bp.queue, 1u);
-  __syncthreads();
-  int64_t t = slot;
-  __syncthreads();
-  int j = 0;
-  while (t < bp.total_tiles) {
-    unsigned int next = 0;
-    if (threadIdx.x == 0) next = atomicAdd(bp.queue, 1u);  // in flight during this tile
-    // tiles are handed out in increasing order, so the job index only advances
-    while (j + 1 < bp.njobs && bp.jobs[j + 1].tile_begin <= t) ++j;
-    const BatchJob& job = bp.jobs[j];
+  const int G = static_cast<int>(gridDim.x);
+  const int64_t b = xcd_block_id(static_cast<int>(blockIdx.x), G, 1);
+  int64_t t = b * bp.tiles_per_block;
+  const int64_t tend = t + bp.tiles_per_block < bp.total_tiles ? t + bp.tiles_per_block : bp.total_tiles;
+  if (t >= tend) return;
+  // last job whose first tile is <= t
+  int lo = 0, hi = bp.njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].tile_begin <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  int j = lo;
+  for (; t < tend; ++t) {
+    while (j + 1 < bp.njobs && jobs[j + 1].tile_begin <= t) ++j;
+    const BatchJob job = jobs[j];  // a register copy for the tile
     const int64_t local = t - job.tile_begin;
     const int bx = static_cast<int>(local % job.tiles_x);
     const int64_t byz = local / job.tiles_x;
     const int by = static_cast<int>(byz % job.tiles_y);
     const int bz = static_cast<int>(byz / job.tiles_y);
     Dispatch::run(job, bx, by, bz, lane, wave);
-    if (threadIdx.x == 0) slot = next;
-    __syncthreads();
-    t = slot;
-    __syncthreads();  // every thread has read the slot before it is rewritten
   }
 }
 
@@ -81,10 +83,12 @@ struct BatchVariant {
   const int8_t* offs;
   int variant;
   int sv, sky;  // streaming tile: 64*sv cells x 4*sky rows
+  int klass;    // the batched kernel that holds this variant (one per shape)
 };
 
 struct BatchKernelEntry {
   int dtype;
+  int klass;
   BatchLaunchFn fn;
   int (*occupancy)();  // resident workgroups per CU
 };
@@ -94,13 +98,13 @@ std::vector<BatchKernelEntry>& batch_kernels();
 
 template <typename S, class Dispatch>
 void launch_batch(const BatchParams& bp, unsigned blocks, hipStream_t s) {
-  hipLaunchKernelGGL((batch_kernel<S, Dispatch>), dim3(blocks), dim3(256), 16, s, bp);
+  hipLaunchKernelGGL((batch_kernel<S, Dispatch>), dim3(blocks), dim3(256), 0, s, bp, bp.jobs);
 }
 
 template <typename S, class Dispatch>
 int batch_occupancy() {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_kernel<S, Dispatch>, 256, 16) != hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_kernel<S, Dispatch>, 256, 0) != hipSuccess)
     n = 1;
   return n > 0 ? n : 1;
 }
