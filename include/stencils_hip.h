@@ -143,6 +143,7 @@ typedef struct sth_op {
 #define STH_GRAPH 2u     /* sth_iterate: replay the steps as a captured hipGraph */
 #define STH_GENERIC 4u   /* force the runtime-offset kernel (testing / tuning) */
 #define STH_NO_TUNE 16u  /* use the default kernel variant; never time variants */
+#define STH_NO_FUSE 32u  /* sth_iterate: no temporal blocking (one generation per launch) */
 
 typedef struct sth_ctx sth_ctx;
 
@@ -186,7 +187,10 @@ int sth_map(sth_ctx* ctx, const sth_array* src, const sth_array* dst,
  * (src/gatherstencil.jl:77-83 + switch, src/array.jl:610-611), ping-ponging
  * buffers a (source) and b (dest) on device.  On return *current is 0 if
  * `a` holds the newest state, 1 if `b` does (the switched wrapper's source).
- * With STH_GRAPH the step pair is captured once into a hipGraph and replayed.
+ * With STH_GRAPH the steps are captured once into hipGraphs and replayed;
+ * Life-like rules then advance two generations per launch in the steady
+ * steps (temporal blocking, STH_NO_FUSE turns it off) and finish with single
+ * steps, so both buffers end exactly as the step-by-step reference leaves them.
  */
 int sth_iterate(sth_ctx* ctx, const sth_array* a, const sth_array* b,
                 const sth_stencil* st, int32_t boundary, const void* padval,

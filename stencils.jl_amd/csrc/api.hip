@@ -16,6 +16,7 @@
 
 #include "batch_kernel.hpp"
 #include "halo_kernel.hpp"
+#include "life2_kernel.hpp"
 #include "life_kernel.hpp"
 #include "registry.hpp"
 #include "stencils_hip.h"
@@ -96,6 +97,9 @@ struct Plan {
   int32_t bsv, bsky; // its streaming tile (64*bsv cells x 4*bsky rows)
   int32_t bklass;    // which batched kernel (one per build-time shape)
   int32_t dtype;     // sth_dtype of the source
+  LaunchFn fn_fused2;     // two generations per launch (steady steps), or null
+  int32_t fused_tiles_y;  // its tile rows
+  int32_t pad2_;
   HaloParams hp;
   void (*halo_fn)(const HaloParams&, hipStream_t);
 };
@@ -332,6 +336,11 @@ int make_plan(const sth_array* src, const sth_array* dst, const sth_stencil* st,
       // map boundaries in registers and refresh the source halo in the same launch
       p.read_halo = 0;
       p.fuse_halo = 1;
+    }
+    if (bc != STH_USE && n[1] >= 3) {
+      // temporal blocking for steady iteration steps (two generations per launch)
+      pl.fn_fused2 = conway ? &launch_life2<4, true> : &launch_life2<4, false>;
+      pl.fused_tiles_y = static_cast<int32_t>((n[1] + 15) / 16);
     }
   } else {
     if (!(flags & STH_GENERIC)) {
@@ -912,18 +921,31 @@ int sth_iterate(sth_ctx* ctx, const sth_array* a, const sth_array* b, const sth_
   // write); steps 1.. use the steady-state kernels.
   if ((rc = run_plan(ab, s))) return rc;
   const int64_t rest = steps - 1;
+  // two generations per launch; buffers then hold generations one apart
+  // from the step-by-step order, so fused launches come in pairs
+  auto run_fused = [](const Plan& pl, hipStream_t st) -> int {
+    MapParams q = pl.p;
+    q.tiles_y = pl.fused_tiles_y;
+    q.fuse_halo = 0;
+    pl.fn_fused2(q, st);
+    STH_HIP(hipGetLastError());
+    return STH_OK;
+  };
   if (!(flags & STH_GRAPH)) {
     for (int64_t i = 1; i < steps; ++i)
       if ((rc = run_plan(i % 2 ? ba : ab, s, true))) return rc;
   } else {
-    // A graph of G = 2U steps starting from buffer b is captured once per
-    // parameter set and replayed; the remainder uses a second cached graph.
-    const int64_t U = rest >= 64 ? 16 : std::max<int64_t>(1, rest / 2);
-    const int64_t G = 2 * U;
-    auto get_graph = [&](int64_t nsteps, hipGraphExec_t* exec) -> int {
+    // Graphs of steady steps (starting from buffer b) are captured once per
+    // parameter set and replayed.  With temporal blocking a unit is two
+    // fused launches (b -> a, a -> b: four generations, buffers back in
+    // step); every iteration ends with at least two single steps, which
+    // leave both buffers and their halos as the step-by-step reference.
+    auto get_graph = [&](const char* tag, int64_t units, int64_t plain, hipGraphExec_t* exec) -> int {
       std::string key(reinterpret_cast<const char*>(&ab), sizeof(Plan));
       key.append(reinterpret_cast<const char*>(&ba), sizeof(Plan));
-      key.append(reinterpret_cast<const char*>(&nsteps), sizeof nsteps);
+      key.append(tag);
+      key.append(reinterpret_cast<const char*>(&units), sizeof units);
+      key.append(reinterpret_cast<const char*>(&plain), sizeof plain);
       std::lock_guard<std::mutex> lk(ctx->mu);
       auto it = ctx->graphs.find(key);
       if (it != ctx->graphs.end()) {
@@ -932,8 +954,11 @@ int sth_iterate(sth_ctx* ctx, const sth_array* a, const sth_array* b, const sth_
       }
       STH_HIP(hipStreamBeginCapture(ctx->capture, hipStreamCaptureModeThreadLocal));
       int r2 = STH_OK;
-      for (int64_t i = 0; i < nsteps && r2 == STH_OK; ++i)
-        r2 = run_plan(i % 2 ? ab : ba, ctx->capture, true);
+      for (int64_t u = 0; u < units && r2 == STH_OK; ++u) {
+        r2 = run_fused(ba, ctx->capture);
+        if (r2 == STH_OK) r2 = run_fused(ab, ctx->capture);
+      }
+      for (int64_t i = 0; i < plain && r2 == STH_OK; ++i) r2 = run_plan(i % 2 ? ab : ba, ctx->capture, true);
       hipGraph_t g = nullptr;
       hipError_t e = hipStreamEndCapture(ctx->capture, &g);
       if (r2) return r2;
@@ -946,16 +971,34 @@ int sth_iterate(sth_ctx* ctx, const sth_array* a, const sth_array* b, const sth_
       *exec = x;
       return STH_OK;
     };
-    const int64_t reps = rest / G, rem = rest % G;
-    if (reps > 0) {
+    const bool fuse = !(flags & STH_NO_FUSE) && ab.fn_fused2 && ba.fn_fused2 && rest >= 6;
+    if (fuse) {
+      const int64_t units = (rest - 2) / 4;       // fused pairs
+      const int64_t plain = rest - 4 * units;     // 2 .. 5 single steps
+      constexpr int64_t FU = 16;                  // units per replayed graph
+      const int64_t reps = units / FU, remu = units % FU;
+      if (reps > 0) {
+        hipGraphExec_t x;
+        if ((rc = get_graph("f", FU, 0, &x))) return rc;
+        for (int64_t i = 0; i < reps; ++i) STH_HIP(hipGraphLaunch(x, s));
+      }
       hipGraphExec_t x;
-      if ((rc = get_graph(G, &x))) return rc;
-      for (int64_t i = 0; i < reps; ++i) STH_HIP(hipGraphLaunch(x, s));
-    }
-    if (rem > 0) {
-      hipGraphExec_t x;
-      if ((rc = get_graph(rem, &x))) return rc;
+      if ((rc = get_graph("f", remu, plain, &x))) return rc;
       STH_HIP(hipGraphLaunch(x, s));
+    } else {
+      const int64_t U = rest >= 64 ? 16 : std::max<int64_t>(1, rest / 2);
+      const int64_t G = 2 * U;
+      const int64_t reps = rest / G, rem = rest % G;
+      if (reps > 0) {
+        hipGraphExec_t x;
+        if ((rc = get_graph("p", 0, G, &x))) return rc;
+        for (int64_t i = 0; i < reps; ++i) STH_HIP(hipGraphLaunch(x, s));
+      }
+      if (rem > 0) {
+        hipGraphExec_t x;
+        if ((rc = get_graph("p", 0, rem, &x))) return rc;
+        STH_HIP(hipGraphLaunch(x, s));
+      }
     }
   }
   STH_HIP(hipGetLastError());

@@ -44,6 +44,7 @@ GRAPH = 2
 GENERIC = 4
 BATCH_SERIAL = 8
 NO_TUNE = 16
+NO_FUSE = 32
 
 
 class SthArray(ctypes.Structure):
