@@ -3,8 +3,8 @@
 // registers, so HBM sees one read and one write per two generations.
 //
 // Each lane owns 16 cells of a row (as in life_kernel) and KY output rows.
-// It loads gen-t rows y0-2 .. y0+KY+1 with two extra cells on each side
-// (x0-2, x0-1, x0+16, x0+17), computes gen t+1 on rows y0-1 .. y0+KY for
+// It loads gen-t rows y0-2 .. y0+KY+1 and takes two extra cells on each side
+// (x0-2, x0-1, x0+16, x0+17) from its neighbouring lanes, computes gen t+1 on rows y0-1 .. y0+KY for
 // its 16 cells plus the two edge cells x0-1 and x0+16, and gen t+2 on its
 // KY rows.  Cells outside the array are boundary-mapped exactly as the
 // reference's getneighbor does (src/array.jl:129-179):
@@ -44,7 +44,7 @@ template <int KY, bool CONWAY, bool EDGE>
 __device__ __forceinline__ void life2_rows(const MapParams& p, const uint8_t* __restrict__ src,
                                            uint8_t* __restrict__ dst, int32_t x0, int32_t y0, bool active,
                                            int32_t xs, const int32_t (&xm)[4], const bool (&xpad)[4],
-                                           bool lvirt, bool rvirt) {
+                                           bool lvirt, bool rvirt, bool fill_l, bool fill_r) {
   const int32_t n1 = static_cast<int32_t>(p.src.n1);
   const int64_t s1 = p.src.s1;
   const int bc = p.bc;
@@ -58,11 +58,14 @@ __device__ __forceinline__ void life2_rows(const MapParams& p, const uint8_t* __
   uint32_t w[NR][4];
   uint32_t e[NR][4];  // gen-t cells x0-2, x0-1, x0+16, x0+17
   bool virt[NR];      // row outside the array
+  bool padrow[NR];
+  const uint8_t* rowp[NR];
+  // 1. every row's 16-byte chunk (all loads in flight together)
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
     const int32_t y = y0 - 2 + r;
     int32_t ym = y;
-    bool padrow = false;
+    padrow[r] = false;
     virt[r] = false;
     if constexpr (EDGE) {
       // rows beyond [-2, n1 + 1] exist only in the last, partial block;
@@ -72,7 +75,7 @@ __device__ __forceinline__ void life2_rows(const MapParams& p, const uint8_t* __
       virt[r] = out;
       if (out) {
         if (remove) {
-          padrow = true;
+          padrow[r] = true;
           ym = 0;
         } else {
           ym = static_cast<int32_t>(map_index(yc, n1, bc));
@@ -81,21 +84,41 @@ __device__ __forceinline__ void life2_rows(const MapParams& p, const uint8_t* __
         ym = yc;
       }
     }
-    const uint8_t* row = src + static_cast<int64_t>(ym) * s1;
-    const uint4 v = *reinterpret_cast<const uint4*>(row + xs);
+    rowp[r] = src + static_cast<int64_t>(ym) * s1;
+    const uint4 v = *reinterpret_cast<const uint4*>(rowp[r] + xs);
     w[r][0] = v.x;
     w[r][1] = v.y;
     w[r][2] = v.z;
     w[r][3] = v.w;
+  }
+  // 2. lanes at a wave or array edge load their outer cells (boundary-mapped)
+  uint32_t fl[NR], fr[NR];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) e[r][k] = xpad[k] ? pb : static_cast<uint32_t>(row[xm[k]]);
-    if (EDGE && padrow) {
+  for (int r = 0; r < NR; ++r) fl[r] = fr[r] = 0;
+  if (fill_l || fill_r) {
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        w[r][k] = padn;
-        e[r][k] = pb;
-      }
+    for (int r = 0; r < NR; ++r) {
+      auto cell = [&](int k) { return xpad[k] ? pb : static_cast<uint32_t>(rowp[r][xm[k]]); };
+      fl[r] = (cell(0) << 16) | (cell(1) << 24);
+      fr[r] = cell(2) | (cell(3) << 8);
     }
+  }
+  // 3. the others take them from the neighbouring lanes' chunks
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    if (EDGE && padrow[r]) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) w[r][k] = padn;
+      fl[r] = (pb << 16) | (pb << 24);
+      fr[r] = pb | (pb << 8);
+    }
+    const uint32_t lw = from_left(w[r][3], fl[r]);
+    const uint32_t rw = from_right(w[r][0], fr[r]);
+    const uint32_t lv = fill_l ? fl[r] : lw, rv = fill_r ? fr[r] : rw;
+    e[r][0] = (lv >> 16) & 0xffu;
+    e[r][1] = lv >> 24;
+    e[r][2] = rv & 0xffu;
+    e[r][3] = (rv >> 8) & 0xffu;
   }
 
   // ---- gen t+1 on rows 1 .. NR-2 (16 cells + the two edge cells)
@@ -184,10 +207,12 @@ __global__ __launch_bounds__(256) void life2_kernel(MapParams p) {
     xm[k] = xpad[k] ? 0 : static_cast<int32_t>(map_index(xo[k], n0, bc));
   }
   const bool lvirt = xs == 0, rvirt = xs + 16 == n0;
+  const bool fill_l = lane == 0 || xs == 0;
+  const bool fill_r = lane == 63 || xs + 16 >= n0 || x0 + 16 >= n0;
   if (y0 >= 2 && y0 + KY + 1 < n1)
-    life2_rows<KY, CONWAY, false>(p, src, dst, x0, y0, active, xs, xm, xpad, lvirt, rvirt);
+    life2_rows<KY, CONWAY, false>(p, src, dst, x0, y0, active, xs, xm, xpad, lvirt, rvirt, fill_l, fill_r);
   else
-    life2_rows<KY, CONWAY, true>(p, src, dst, x0, y0, active, xs, xm, xpad, lvirt, rvirt);
+    life2_rows<KY, CONWAY, true>(p, src, dst, x0, y0, active, xs, xm, xpad, lvirt, rvirt, fill_l, fill_r);
 }
 
 template <int KY, bool CONWAY>

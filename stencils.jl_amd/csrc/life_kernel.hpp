@@ -25,6 +25,18 @@
 
 namespace sth {
 
+// Cross-lane moves (DPP wave shifts, full rate): lane i receives lane i - 1
+// (from_left) or lane i + 1 (from_right); the lane with no such neighbour
+// (0, resp. 63) receives `fill`.
+__device__ __forceinline__ uint32_t from_left(uint32_t v, uint32_t fill) {
+  return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(static_cast<int>(fill), static_cast<int>(v),
+                                                          0x138 /* wave_shr:1 */, 0xf, 0xf, false));
+}
+__device__ __forceinline__ uint32_t from_right(uint32_t v, uint32_t fill) {
+  return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(static_cast<int>(fill), static_cast<int>(v),
+                                                          0x130 /* wave_shl:1 */, 0xf, 0xf, false));
+}
+
 __device__ __forceinline__ uint32_t norm01(uint32_t x) {
   // 0x01 in every byte that is nonzero, 0x00 elsewhere.
   const uint32_t t = ((x & 0x7f7f7f7fu) + 0x7f7f7f7fu) | x;
@@ -58,7 +70,7 @@ template <int KY, bool CONWAY, bool NORM, bool EDGE>
 __device__ __forceinline__ void life_rows(const MapParams& p, const uint8_t* __restrict__ src,
                                           uint8_t* __restrict__ dst, int32_t x0, int32_t y0,
                                           bool active, int32_t xs, int32_t xlm, int32_t xrm,
-                                          bool lpad, bool rpad) {
+                                          bool lpad, bool rpad, bool fill_l, bool fill_r) {
   const int32_t n1 = static_cast<int32_t>(p.src.n1);
   const int64_t s1 = p.src.s1;
   const int bc = p.bc;
@@ -71,14 +83,14 @@ __device__ __forceinline__ void life_rows(const MapParams& p, const uint8_t* __r
   uint4 raw[NR];
   uint32_t lb[NR], rb[NR];
   bool padrow[NR];
+  const uint8_t* rowp[NR];
   if constexpr (!EDGE) {
     const uint8_t* row = src + static_cast<int64_t>(y0 - 1) * s1;
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
       padrow[r] = false;
+      rowp[r] = row;
       raw[r] = *reinterpret_cast<const uint4*>(row + xs);
-      lb[r] = row[xlm];
-      rb[r] = row[xrm];
       row += s1;
     }
   } else {
@@ -94,11 +106,29 @@ __device__ __forceinline__ void life_rows(const MapParams& p, const uint8_t* __r
       const int32_t yr = yc < 0 ? -yc : (yc >= n1 ? 2 * n1 - 2 - yc : yc);
       const int32_t ym = halo ? yc : (wrap ? yw : (refl ? yr : (out ? 0 : yc)));
       padrow[r] = !halo && !wrap && !refl && out;
-      const uint8_t* row = src + static_cast<int64_t>(ym) * s1;
-      raw[r] = *reinterpret_cast<const uint4*>(row + xs);
-      lb[r] = row[xlm];
-      rb[r] = row[xrm];
+      rowp[r] = src + static_cast<int64_t>(ym) * s1;
+      raw[r] = *reinterpret_cast<const uint4*>(rowp[r] + xs);
     }
+  }
+  // The edge bytes (x0 - 1, x0 + 16) come from the neighbouring lanes'
+  // chunks; lanes at a wave or array edge (fill_l / fill_r) load them,
+  // boundary-mapped, after every chunk load is in flight.
+  uint32_t fl[NR], fr[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) fl[r] = fr[r] = 0;
+  if (fill_l || fill_r) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      fl[r] = static_cast<uint32_t>(rowp[r][xlm]) << 24;
+      fr[r] = rowp[r][xrm];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const uint32_t lw = from_left(raw[r].w, fl[r]);
+    const uint32_t rw = from_right(raw[r].x, fr[r]);
+    lb[r] = fill_l ? (fl[r] >> 24) : (lw >> 24);
+    rb[r] = fill_r ? fr[r] : (rw & 0xffu);
   }
 
   auto hsum = [&](int r, uint32_t* w, uint32_t* h) {
@@ -190,10 +220,14 @@ __global__ __launch_bounds__(256) void life_kernel(MapParams p) {
   const int32_t xlm = halo ? xl : (lpad ? 0 : static_cast<int32_t>(map_index(xl, n0, bc)));
   const int32_t xrm = halo ? xr : (rpad ? 0 : static_cast<int32_t>(map_index(xr, n0, bc)));
 
+  // lanes whose neighbour chunk is outside this wave or the array load the
+  // edge byte themselves (every lane takes part in the cross-lane moves)
+  const bool fill_l = lane == 0 || xs == 0;
+  const bool fill_r = lane == 63 || xs + 16 >= n0 || x0 + 16 >= n0;
   if (y0 >= 1 && y0 + KY < n1)
-    life_rows<KY, CONWAY, NORM, false>(p, src, dst, x0, y0, active, xs, xlm, xrm, lpad, rpad);
+    life_rows<KY, CONWAY, NORM, false>(p, src, dst, x0, y0, active, xs, xlm, xrm, lpad, rpad, fill_l, fill_r);
   else
-    life_rows<KY, CONWAY, NORM, true>(p, src, dst, x0, y0, active, xs, xlm, xrm, lpad, rpad);
+    life_rows<KY, CONWAY, NORM, true>(p, src, dst, x0, y0, active, xs, xlm, xrm, lpad, rpad, fill_l, fill_r);
 
   if (p.fuse_halo && active) {
     // update_boundary! of the source: halo cell (x, y) takes the value of the
