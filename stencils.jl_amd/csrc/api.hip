@@ -339,8 +339,13 @@ int make_plan(const sth_array* src, const sth_array* dst, const sth_stencil* st,
     }
     if (bc != STH_USE && n[1] >= 3) {
       // temporal blocking for steady iteration steps (two generations per launch)
-      pl.fn_fused2 = conway ? &launch_life2<4, true> : &launch_life2<4, false>;
-      pl.fused_tiles_y = static_cast<int32_t>((n[1] + 15) / 16);
+      int ky2 = 8;  // rows per wave of the fused kernel (measured best; STH_LIFE2_KY: 2, 4 or 8)
+      if (const char* e = std::getenv("STH_LIFE2_KY")) ky2 = std::atoi(e);
+      if (ky2 != 2 && ky2 != 4) ky2 = 8;
+      pl.fn_fused2 = ky2 == 2 ? (conway ? &launch_life2<2, true> : &launch_life2<2, false>)
+                   : ky2 == 8 ? (conway ? &launch_life2<8, true> : &launch_life2<8, false>)
+                              : (conway ? &launch_life2<4, true> : &launch_life2<4, false>);
+      pl.fused_tiles_y = static_cast<int32_t>((n[1] + 4 * ky2 - 1) / (4 * ky2));
     }
   } else {
     if (!(flags & STH_GENERIC)) {
