@@ -111,7 +111,17 @@ struct Variant {
   LaunchFn fn_steady = nullptr;
   const char* name = "";
   int32_t tiles_x = 0, tiles_y = 0, tiles_z = 0;
+  int32_t lpr_shift = 6;  // streaming kernels: log2 of the lanes per row
 };
+
+// Lanes per row of the streaming kernels: the fewest (a power of two, 8..64)
+// that cover a row of n0 cells at V cells per lane.
+int lpr_shift_for(int64_t n0, int v) {
+  const int64_t lanes = (n0 + v - 1) / v;
+  int sh = 3;
+  while (sh < 6 && (int64_t(1) << sh) < lanes) ++sh;
+  return sh;
+}
 
 void apply_variant(Plan& pl, const Variant& v) {
   pl.fn = v.fn;
@@ -120,6 +130,7 @@ void apply_variant(Plan& pl, const Variant& v) {
   pl.p.tiles_x = v.tiles_x;
   pl.p.tiles_y = v.tiles_y;
   pl.p.tiles_z = v.tiles_z;
+  pl.p.lpr_shift = v.lpr_shift;
   if (pl.p.src.n0 * pl.p.src.n1 * pl.p.src.n2 == 0) pl.fn = pl.fn_steady = nullptr;  // nothing to launch
 }
 
@@ -361,8 +372,10 @@ int make_plan(const sth_array* src, const sth_array* dst, const sth_stencil* st,
         v.fn = v.fn_steady = e.fn;
         v.name = e.name;
         if (e.cfg == CFG_STREAM) {
-          v.tiles_x = static_cast<int32_t>((n[0] + 64 * e.sv - 1) / (64 * e.sv));
-          v.tiles_y = static_cast<int32_t>((n[1] + 4 * e.sky - 1) / (4 * e.sky));
+          v.lpr_shift = lpr_shift_for(n[0], e.sv);
+          const int64_t lpr = int64_t(1) << v.lpr_shift, rpw = 64 / lpr;
+          v.tiles_x = static_cast<int32_t>((n[0] + lpr * e.sv - 1) / (lpr * e.sv));
+          v.tiles_y = static_cast<int32_t>((n[1] + 4 * e.sky * rpw - 1) / (4 * e.sky * rpw));
           if (n[2] > (int64_t(1) << 31) / 2) continue;
           v.tiles_z = static_cast<int32_t>(n[2]);
         } else {
@@ -620,8 +633,10 @@ int launch_batched(sth_ctx* ctx, const Plan* const* plans, int32_t n, bool seria
     bj.padval = pl.p.padval;
     bj.op = pl.p.op;
     bj.variant = pl.bvariant;
-    bj.tiles_x = static_cast<int32_t>((pl.p.src.n0 + 64 * pl.bsv - 1) / (64 * pl.bsv));
-    bj.tiles_y = static_cast<int32_t>((pl.p.src.n1 + 4 * pl.bsky - 1) / (4 * pl.bsky));
+    bj.lpr_shift = lpr_shift_for(pl.p.src.n0, pl.bsv);
+    const int64_t lpr = int64_t(1) << bj.lpr_shift, rpw = 64 / lpr;
+    bj.tiles_x = static_cast<int32_t>((pl.p.src.n0 + lpr * pl.bsv - 1) / (lpr * pl.bsv));
+    bj.tiles_y = static_cast<int32_t>((pl.p.src.n1 + 4 * pl.bsky * rpw - 1) / (4 * pl.bsky * rpw));
     bj.tiles_z = static_cast<int32_t>(pl.p.src.n2);
     groups[pl.dtype * 64 + pl.bklass].push_back(bj);
   }

@@ -27,6 +27,8 @@ struct BatchJob {
   OpParams op;
   int32_t variant;  // case of the generated dispatch switch
   int32_t tiles_x, tiles_y, tiles_z;
+  int32_t lpr_shift;   // streaming body: log2 of the lanes per row
+  int32_t pad_;
   int64_t tile_begin;  // first tile of this job in the flattened index space
 };
 

@@ -188,7 +188,7 @@ struct MapParams {
   int32_t xcd_remap;                  // 1: remap block ids so each XCD owns a contiguous range
   int32_t fuse_halo;                  // Life kernel: read the interior only and refresh the
                                       // source halo in the same launch (update_boundary!)
-  int32_t pad0_;
+  int32_t lpr_shift;                  // streaming kernels: log2 of the lanes per row (3..6)
   // runtime stencils: packed (dx, dy, dz) as int8 each in the low 24 bits
   int32_t offs[kMaxRuntimeL];
   // Kernel weights in the accumulator type (f64 uses both halves).

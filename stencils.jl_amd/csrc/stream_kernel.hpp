@@ -3,9 +3,12 @@
 //
 // Work split (one workgroup = 4 wave64s):
 //   lane  -> V consecutive cells along dim 1 (one 16-byte vector for the
-//            natural V of the element type),
-//   wave  -> KY consecutive rows along dim 2,
-//   block -> 64 V cells x 4 KY rows of one dim-3 plane.
+//            natural V of the element type) and KY consecutive rows along
+//            dim 2,
+//   wave  -> LPR = 2^lpr_shift lanes along a row x 64 / LPR lane rows (the
+//            host picks LPR from the array width so narrow arrays keep every
+//            lane busy),
+//   block -> LPR V cells x 4 (64 / LPR) KY rows of one dim-3 plane.
 // Each lane loads every source row its KY output rows need -- the vector
 // part plus RX cells on either side -- into registers, folds the stencil for
 // its KY x V outputs in offset order (the reference's left fold,
@@ -70,11 +73,16 @@ __device__ __forceinline__ void stream_body(const P& p, int bx, int by, int bz, 
   constexpr int NX = V + 2 * RX, NY = KY + 2 * RY, NZ = 2 * RZ + 1;
 
   const int64_t n0 = p.src.n0, n1 = p.src.n1, n2 = p.src.n2;
-  const int64_t y0 = (static_cast<int64_t>(by) * 4 + wave) * KY;
+  // A wave covers 2^lsh lanes of a row and 64 >> lsh rows side by side
+  // (narrow arrays keep every lane busy); each lane owns KY consecutive rows.
+  const int lsh = p.lpr_shift;
+  const int LPR = 1 << lsh, RPW = 64 >> lsh;
+  const int64_t wy0 = (static_cast<int64_t>(by) * 4 + wave) * KY * RPW;  // first row of the wave
   const int64_t z0 = bz;
-  if (y0 >= n1) return;  // uniform: the wave has no rows (no barrier follows)
-  const int64_t wx0 = static_cast<int64_t>(bx) * 64 * V;  // first cell of the wave
-  const int64_t x0 = wx0 + static_cast<int64_t>(lane) * V;
+  if (wy0 >= n1) return;  // uniform: the wave has no rows (no barrier follows)
+  const int64_t y0 = wy0 + static_cast<int64_t>(lane >> lsh) * KY;        // first row of the lane
+  const int64_t wx0 = static_cast<int64_t>(bx) * LPR * V;                  // first cell of the wave
+  const int64_t x0 = wx0 + static_cast<int64_t>(lane & (LPR - 1)) * V;
   const bool halo = p.read_halo != 0;
   const int64_t h0 = halo ? p.src.h0 : 0, h1 = halo ? p.src.h1 : 0, h2 = halo ? p.src.h2 : 0;
   const int bc = p.bc;
@@ -83,8 +91,9 @@ __device__ __forceinline__ void stream_body(const P& p, int bx, int by, int bz, 
 
   Val v[NZ][NY][NX];
 
-  const bool rows_direct = y0 - RY >= -h1 && y0 + KY - 1 + RY < n1 + h1 && z0 - RZ >= -h2 && z0 + RZ < n2 + h2;
-  const bool wave_x_direct = wx0 - RX >= -h0 && wx0 + 64 * V + RX <= n0 + h0;
+  const bool rows_direct = wy0 - RY >= -h1 && wy0 + RPW * KY - 1 + RY < n1 + h1 && z0 - RZ >= -h2 &&
+                           z0 + RZ < n2 + h2;
+  const bool wave_x_direct = wx0 - RX >= -h0 && wx0 + LPR * V + RX <= n0 + h0;
   if (rows_direct && wave_x_direct) {
     // ---- interior wave: straight rows, one vector + 2 RX cells per row
 #pragma unroll
