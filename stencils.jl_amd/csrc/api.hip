@@ -16,6 +16,7 @@
 
 #include "batch_kernel.hpp"
 #include "halo_kernel.hpp"
+#include "heat2_kernel.hpp"
 #include "life2_kernel.hpp"
 #include "life_kernel.hpp"
 #include "registry.hpp"
@@ -25,6 +26,10 @@ namespace sth {
 
 std::vector<KernelEntry>& registry() {
   static std::vector<KernelEntry> r;
+  return r;
+}
+std::vector<Fused2Entry>& fused2_registry() {
+  static std::vector<Fused2Entry> r;
   return r;
 }
 std::vector<BatchVariant>& batch_variants() {
@@ -95,11 +100,10 @@ struct Plan {
   int32_t halo;
   int32_t bvariant;  // case of the batched kernel for this call, -1 if none
   int32_t bsv, bsky; // its streaming tile (64*bsv cells x 4*bsky rows)
-  int32_t bklass;    // which batched kernel (one per build-time shape)
+  int32_t bklass;    // which batched kernel (one per (shape, op) variant)
   int32_t dtype;     // sth_dtype of the source
-  LaunchFn fn_fused2;     // two generations per launch (steady steps), or null
-  int32_t fused_tiles_y;  // its tile rows
-  int32_t pad2_;
+  LaunchFn fn_fused2;     // two steps per launch (steady steps), or null
+  int32_t fused_tiles_x, fused_tiles_y, fused_tiles_z;  // its grid
   HaloParams hp;
   void (*halo_fn)(const HaloParams&, hipStream_t);
 };
@@ -356,7 +360,9 @@ int make_plan(const sth_array* src, const sth_array* dst, const sth_stencil* st,
       pl.fn_fused2 = ky2 == 2 ? (conway ? &launch_life2<2, true> : &launch_life2<2, false>)
                    : ky2 == 8 ? (conway ? &launch_life2<8, true> : &launch_life2<8, false>)
                               : (conway ? &launch_life2<4, true> : &launch_life2<4, false>);
+      pl.fused_tiles_x = static_cast<int32_t>((n[0] + 1023) / 1024);
       pl.fused_tiles_y = static_cast<int32_t>((n[1] + 4 * ky2 - 1) / (4 * ky2));
+      pl.fused_tiles_z = 1;
     }
   } else {
     if (!(flags & STH_GENERIC)) {
@@ -430,6 +436,27 @@ int make_plan(const sth_array* src, const sth_array* dst, const sth_stencil* st,
   apply_variant(pl, cands[0]);
   if (cands_out) *cands_out = std::move(cands);
 
+  // two steps per launch for 3-D radius-1 catalogue stencils (Wrap / Remove)
+  // on arrays large enough for the z-marching kernel to beat two streaming
+  // steps (STH_FUSE3D_MIN_CELLS overrides the threshold)
+  static const int64_t fuse3d_min = [] {
+    const char* e = std::getenv("STH_FUSE3D_MIN_CELLS");
+    return e ? std::atoll(e) : (int64_t(1) << 27);
+  }();
+  if (!pl.fn_fused2 && !(flags & (STH_NO_FUSE | STH_GENERIC)) && src->ndim == 3 && st->ndim == 3 &&
+      (bc == STH_WRAP || bc == STH_REMOVE) && n[0] >= 3 && n[1] >= 3 && n[2] >= 3 &&
+      n[0] * n[1] * n[2] >= fuse3d_min) {
+    for (const auto& e : fused2_registry()) {
+      if (e.dtype != sdt || e.op != op->kind || e.L != L || e.ndim != st->ndim) continue;
+      if (!offsets_equal(e.offs, L, offs3)) continue;
+      pl.fn_fused2 = e.fn;
+      pl.fused_tiles_x = static_cast<int32_t>((n[0] + kHeat2TX - 1) / kHeat2TX);
+      pl.fused_tiles_y = static_cast<int32_t>((n[1] + kHeat2TY - 1) / kHeat2TY);
+      pl.fused_tiles_z = static_cast<int32_t>((n[2] + kHeat2ZC - 1) / kHeat2ZC);
+      break;
+    }
+  }
+
   // the batched kernel's variant for this (dtype, op, offsets), if any
   pl.dtype = sdt;
   pl.bvariant = -1;
@@ -479,6 +506,28 @@ int run_plan(const Plan& pl, hipStream_t s, bool steady = false) {
   if (f) f(pl.p, s);
   STH_HIP(hipGetLastError());
   return STH_OK;
+}
+
+// Two steps in one launch (pl.fn_fused2, steady state).  The fused kernels
+// map every boundary themselves and write only the destination's interior.
+int run_fused2(const Plan& pl, hipStream_t s) {
+  MapParams q = pl.p;
+  q.tiles_x = pl.fused_tiles_x;
+  q.tiles_y = pl.fused_tiles_y;
+  q.tiles_z = pl.fused_tiles_z;
+  q.read_halo = 0;
+  q.fuse_halo = 0;
+  pl.fn_fused2(q, s);
+  STH_HIP(hipGetLastError());
+  return STH_OK;
+}
+
+// A job that fills the device by itself (a full wave of resident workgroups)
+// runs as its own launch of the tuned single-array kernel; batching pays off
+// for the jobs whose separate launches would be overhead-bound.
+bool runs_alone(const Plan& pl, int cus) {
+  const int64_t own_tiles = static_cast<int64_t>(pl.p.tiles_x) * pl.p.tiles_y * pl.p.tiles_z;
+  return own_tiles >= 8 * static_cast<int64_t>(cus);
 }
 
 }  // namespace
@@ -612,10 +661,10 @@ int tune_pick(sth_ctx* ctx, const Plan& pl, const std::vector<Variant>& c, const
 // rest run as single launches first.  `serial_only` runs every call alone.
 int launch_batched(sth_ctx* ctx, const Plan* const* plans, int32_t n, bool serial_only, hipStream_t s) {
   int rc = STH_OK;
-  std::map<int, std::vector<BatchJob>> groups;
+  std::map<int64_t, std::vector<BatchJob>> groups;
   for (int32_t i = 0; i < n; ++i) {
     const Plan& pl = *plans[i];
-    if (serial_only || pl.bvariant < 0 || !pl.fn) {
+    if (serial_only || pl.bvariant < 0 || !pl.fn || runs_alone(pl, ctx->cus)) {
       if (pl.fn || pl.halo)
         if ((rc = run_plan(pl, s))) return rc;
       continue;
@@ -638,13 +687,14 @@ int launch_batched(sth_ctx* ctx, const Plan* const* plans, int32_t n, bool seria
     bj.tiles_x = static_cast<int32_t>((pl.p.src.n0 + lpr * pl.bsv - 1) / (lpr * pl.bsv));
     bj.tiles_y = static_cast<int32_t>((pl.p.src.n1 + 4 * pl.bsky * rpw - 1) / (4 * pl.bsky * rpw));
     bj.tiles_z = static_cast<int32_t>(pl.p.src.n2);
-    groups[pl.dtype * 64 + pl.bklass].push_back(bj);
+    groups[static_cast<int64_t>(pl.dtype) * 65536 + pl.bklass].push_back(bj);
   }
   for (auto& kv : groups) {
     const BatchKernelEntry* ke = nullptr;
     for (const auto& e : batch_kernels())
-      if (e.dtype * 64 + e.klass == kv.first) ke = &e;
-    if (!ke) return fail(STH_EDEVICE, "no batched kernel for dtype %d class %d", kv.first / 64, kv.first % 64);
+      if (static_cast<int64_t>(e.dtype) * 65536 + e.klass == kv.first) ke = &e;
+    if (!ke) return fail(STH_EDEVICE, "no batched kernel for dtype %d variant %d", static_cast<int>(kv.first / 65536),
+                           static_cast<int>(kv.first % 65536));
     std::vector<BatchJob>& bj = kv.second;
     int64_t total = 0;
     for (auto& j : bj) {
@@ -652,11 +702,10 @@ int launch_batched(sth_ctx* ctx, const Plan* const* plans, int32_t n, bool seria
       total += static_cast<int64_t>(j.tiles_x) * j.tiles_y * j.tiles_z;
     }
     if (total == 0) continue;
-    // strips of consecutive tiles per workgroup: enough workgroups for about
-    // four waves of residency, at most 8 tiles each (amortises the job search)
-    const int64_t resident = static_cast<int64_t>(ke->occupancy()) * ctx->cus;
-    int64_t tpb = std::max<int64_t>(1, std::min<int64_t>(8, total / (4 * resident)));
-    if (const char* e = std::getenv("STH_BATCH_TPB")) tpb = std::max(1, std::atoi(e));  // tuning knob
+    // one tile per workgroup (measured fastest: a workgroup's tiles run one
+    // after another with no overlap inside a wave); STH_BATCH_TPB sets strips
+    int64_t tpb = 1;
+    if (const char* e = std::getenv("STH_BATCH_TPB")) tpb = std::max(1, std::atoi(e));
     const int64_t blocks = (total + tpb - 1) / tpb;
     if (blocks > (int64_t(1) << 31) - 1)
       return fail(STH_ENOTSUP, "batch of %lld tiles is too large", static_cast<long long>(total));
@@ -941,31 +990,34 @@ int sth_iterate(sth_ctx* ctx, const sth_array* a, const sth_array* b, const sth_
   // write); steps 1.. use the steady-state kernels.
   if ((rc = run_plan(ab, s))) return rc;
   const int64_t rest = steps - 1;
-  // two generations per launch; buffers then hold generations one apart
-  // from the step-by-step order, so fused launches come in pairs
-  auto run_fused = [](const Plan& pl, hipStream_t st) -> int {
-    MapParams q = pl.p;
-    q.tiles_y = pl.fused_tiles_y;
-    q.fuse_halo = 0;
-    pl.fn_fused2(q, st);
-    STH_HIP(hipGetLastError());
-    return STH_OK;
+  // Temporal blocking: two steps per launch.  Buffers then hold generations
+  // one apart from the step-by-step order, so fused launches come in pairs
+  // (a unit: b -> a, a -> b, four steps, buffers back in step), and every
+  // iteration ends with at least two single steps, which leave both buffers
+  // and their halos as the step-by-step reference does.
+  const bool fuse = !(flags & STH_NO_FUSE) && ab.fn_fused2 && ba.fn_fused2 && rest >= 6;
+  const int64_t units = fuse ? (rest - 2) / 4 : 0;  // fused pairs
+  const int64_t tail = rest - 4 * units;            // single steps (2 .. 5 when fused)
+  auto enqueue = [&](hipStream_t st, int64_t nunits, int64_t nplain) -> int {
+    int r2 = STH_OK;
+    for (int64_t u = 0; u < nunits && r2 == STH_OK; ++u) {
+      r2 = run_fused2(ba, st);
+      if (r2 == STH_OK) r2 = run_fused2(ab, st);
+    }
+    for (int64_t i = 0; i < nplain && r2 == STH_OK; ++i) r2 = run_plan(i % 2 ? ab : ba, st, true);
+    return r2;
   };
   if (!(flags & STH_GRAPH)) {
-    for (int64_t i = 1; i < steps; ++i)
-      if ((rc = run_plan(i % 2 ? ba : ab, s, true))) return rc;
+    if ((rc = enqueue(s, units, tail))) return rc;
   } else {
     // Graphs of steady steps (starting from buffer b) are captured once per
-    // parameter set and replayed.  With temporal blocking a unit is two
-    // fused launches (b -> a, a -> b: four generations, buffers back in
-    // step); every iteration ends with at least two single steps, which
-    // leave both buffers and their halos as the step-by-step reference.
-    auto get_graph = [&](const char* tag, int64_t units, int64_t plain, hipGraphExec_t* exec) -> int {
+    // parameter set and replayed.
+    auto get_graph = [&](const char* tag, int64_t nunits, int64_t nplain, hipGraphExec_t* exec) -> int {
       std::string key(reinterpret_cast<const char*>(&ab), sizeof(Plan));
       key.append(reinterpret_cast<const char*>(&ba), sizeof(Plan));
       key.append(tag);
-      key.append(reinterpret_cast<const char*>(&units), sizeof units);
-      key.append(reinterpret_cast<const char*>(&plain), sizeof plain);
+      key.append(reinterpret_cast<const char*>(&nunits), sizeof nunits);
+      key.append(reinterpret_cast<const char*>(&nplain), sizeof nplain);
       std::lock_guard<std::mutex> lk(ctx->mu);
       auto it = ctx->graphs.find(key);
       if (it != ctx->graphs.end()) {
@@ -973,12 +1025,7 @@ int sth_iterate(sth_ctx* ctx, const sth_array* a, const sth_array* b, const sth_
         return STH_OK;
       }
       STH_HIP(hipStreamBeginCapture(ctx->capture, hipStreamCaptureModeThreadLocal));
-      int r2 = STH_OK;
-      for (int64_t u = 0; u < units && r2 == STH_OK; ++u) {
-        r2 = run_fused(ba, ctx->capture);
-        if (r2 == STH_OK) r2 = run_fused(ab, ctx->capture);
-      }
-      for (int64_t i = 0; i < plain && r2 == STH_OK; ++i) r2 = run_plan(i % 2 ? ab : ba, ctx->capture, true);
+      const int r2 = enqueue(ctx->capture, nunits, nplain);
       hipGraph_t g = nullptr;
       hipError_t e = hipStreamEndCapture(ctx->capture, &g);
       if (r2) return r2;
@@ -991,11 +1038,8 @@ int sth_iterate(sth_ctx* ctx, const sth_array* a, const sth_array* b, const sth_
       *exec = x;
       return STH_OK;
     };
-    const bool fuse = !(flags & STH_NO_FUSE) && ab.fn_fused2 && ba.fn_fused2 && rest >= 6;
     if (fuse) {
-      const int64_t units = (rest - 2) / 4;       // fused pairs
-      const int64_t plain = rest - 4 * units;     // 2 .. 5 single steps
-      constexpr int64_t FU = 16;                  // units per replayed graph
+      constexpr int64_t FU = 16;  // units per replayed graph
       const int64_t reps = units / FU, remu = units % FU;
       if (reps > 0) {
         hipGraphExec_t x;
@@ -1003,7 +1047,7 @@ int sth_iterate(sth_ctx* ctx, const sth_array* a, const sth_array* b, const sth_
         for (int64_t i = 0; i < reps; ++i) STH_HIP(hipGraphLaunch(x, s));
       }
       hipGraphExec_t x;
-      if ((rc = get_graph("f", remu, plain, &x))) return rc;
+      if ((rc = get_graph("f", remu, tail, &x))) return rc;
       STH_HIP(hipGraphLaunch(x, s));
     } else {
       const int64_t U = rest >= 64 ? 16 : std::max<int64_t>(1, rest / 2);

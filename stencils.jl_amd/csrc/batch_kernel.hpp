@@ -1,5 +1,5 @@
 // Batched gather: many independent mapstencil calls of mixed sizes, shapes
-// and reductions in one launch per (element type, stencil shape).
+// and reductions in one launch per (element type, stencil shape, op).
 //
 // The host flattens the tiles of every job into one index space (job j owns
 // tiles [tile_begin_j, tile_begin_{j+1})) and launches one workgroup per
@@ -8,8 +8,9 @@
 // atomics or barriers (a single device-wide atomic tile counter measured much
 // slower: same-address atomics serialise in L2).  A workgroup finds its first
 // job by binary search over the job table (uniform, scalar loads) and walks
-// forward.  Each tile runs the streaming body of its job's (shape, op)
-// variant, selected by a switch generated per element type and class.
+// forward.  Each tile runs the streaming body of the kernel's (shape, op)
+// variant: one small kernel per variant keeps the code in the instruction
+// cache (a kernel switching over a shape's ops measured up to 40 % slower).
 // Results are bit-identical to one sth_map call per job (same fold
 // expressions per cell).
 #pragma once
@@ -39,11 +40,9 @@ struct BatchParams {
   int64_t total_tiles;
 };
 
-// `Dispatch::run(job, bx, by, bz, lane, wave)` runs the streaming body of
-// job.variant; it is generated per storage type and force-inlined into the
-// kernel.  (An out-of-line dispatch function of this size gets long-branch
-// sequences that reuse the return-address registers, so the kernel must not
-// call one.)
+// `Dispatch::run(job, bx, by, bz, lane, wave)` runs the streaming body of the
+// kernel's variant; it is generated per (storage type, shape, op) and
+// force-inlined into the kernel, which makes no calls.
 template <typename S, class Dispatch>
 __global__ __launch_bounds__(256) void batch_kernel(BatchParams bp, const BatchJob* __restrict__ jobs) {
   // `jobs` is read-only and aliases nothing the kernel writes, so its fields
@@ -66,11 +65,13 @@ __global__ __launch_bounds__(256) void batch_kernel(BatchParams bp, const BatchJ
   for (; t < tend; ++t) {
     while (j + 1 < bp.njobs && jobs[j + 1].tile_begin <= t) ++j;
     const BatchJob job = jobs[j];  // a register copy for the tile
-    const int64_t local = t - job.tile_begin;
-    const int bx = static_cast<int>(local % job.tiles_x);
-    const int64_t byz = local / job.tiles_x;
-    const int by = static_cast<int>(byz % job.tiles_y);
-    const int bz = static_cast<int>(byz / job.tiles_y);
+    // a job's own tile count fits in 32 bits (checked on the host)
+    const uint32_t local = static_cast<uint32_t>(t - job.tile_begin);
+    const uint32_t tx = static_cast<uint32_t>(job.tiles_x), ty = static_cast<uint32_t>(job.tiles_y);
+    const int bx = static_cast<int>(local % tx);
+    const uint32_t byz = local / tx;
+    const int by = static_cast<int>(byz % ty);
+    const int bz = static_cast<int>(byz / ty);
     Dispatch::run(job, bx, by, bz, lane, wave);
   }
 }
@@ -85,7 +86,7 @@ struct BatchVariant {
   const int8_t* offs;
   int variant;
   int sv, sky;  // streaming tile: 64*sv cells x 4*sky rows
-  int klass;    // the batched kernel that holds this variant (one per shape)
+  int klass;    // the batched kernel that runs this variant (one per variant)
 };
 
 struct BatchKernelEntry {

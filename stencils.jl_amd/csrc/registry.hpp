@@ -51,8 +51,21 @@ struct KernelEntry {
 
 std::vector<KernelEntry>& registry();
 
+// Kernels that advance an iteration by two steps per launch (temporal
+// blocking), matched like KernelEntry.
+struct Fused2Entry {
+  int dtype;
+  int op;
+  int L;
+  int ndim;
+  const int8_t* offs;
+  LaunchFn fn;
+};
+std::vector<Fused2Entry>& fused2_registry();
+
 struct Registrar {
   explicit Registrar(const KernelEntry& e) { registry().push_back(e); }
+  explicit Registrar(const Fused2Entry& e) { fused2_registry().push_back(e); }
 };
 
 }  // namespace sth

@@ -12,10 +12,13 @@
 //                 a request that does not fit holds back the ones behind it,
 //                 so large requests are not starved);
 //   3. advance -- K = min(round_steps, fewest remaining steps) steps; each
-//                 step is one persistent launch over every active request
+//                 step is one batched launch over the active requests
 //                 (requests join and leave only between rounds), buffers
-//                 ping-pong per request; requests that reach their step count
-//                 get their state copied to `result` and an event recorded.
+//                 ping-pong per request.  Requests that fill the device by
+//                 themselves and have a two-step kernel run their own
+//                 launches instead, two steps each, up to 4 x round_steps
+//                 steps per round.  Requests that reach their step count get
+//                 their state copied to `result` and an event recorded.
 // Launches are stream-ordered, so a freed buffer is reused only by work
 // enqueued after the copy that last read it.  The host runs at most
 // kBatchSlots launches ahead of the device (the batched launcher's job-table
@@ -204,26 +207,71 @@ struct sth_server {
   }
 
   // --- 3. one round (called without `mu`: only the scheduler touches `active`)
+  // A request that runs as launches of its own (it fills the device by
+  // itself) and has a two-step kernel: it advances two steps per launch and
+  // more steps per round than the shared batch.
+  bool solo(const Req& q) const {
+    return q.plan[0].fn_fused2 && q.plan[1].fn_fused2 && sth::runs_alone(q.plan[0], ctx->cus);
+  }
+
+  // `n` steps of a solo request.  Fused launches leave the buffers holding
+  // generations one apart from the step-by-step order, which later steps do
+  // not see (each reads only its source's interior, refreshing halos first);
+  // a request's last two steps are always single steps, so its final state,
+  // halo included, is the step-by-step one.
+  int advance_solo(Req& q, int64_t n, int64_t* launches) {
+    const int64_t rem = q.r.steps - q.done;
+    const int64_t fused = n == rem ? std::max<int64_t>(0, (n - 2) / 2) : n / 2;
+    for (int64_t i = 0; i < fused; ++i) {
+      if (int rc = sth::run_fused2(q.plan[q.cur], stream)) return rc;
+      q.cur ^= 1;
+      q.done += 2;
+      *launches += 1;
+    }
+    for (int64_t i = 2 * fused; i < n; ++i) {
+      if (int rc = sth::run_plan(q.plan[q.cur], stream)) return rc;
+      q.cur ^= 1;
+      q.done += 1;
+      *launches += 1;
+    }
+    return STH_OK;
+  }
+
   int advance(int64_t* launches) {
-    int64_t K = cfg.round_steps > 0 ? cfg.round_steps : 8;
-    for (auto& q : active) K = std::min(K, q->r.steps - q->done);
+    const int64_t R = cfg.round_steps > 0 ? cfg.round_steps : 8;
     const bool serial = (cfg.flags & STH_SERVER_SERIAL) != 0;
-    if (serial) K = active.empty() ? 0 : active.front()->r.steps - active.front()->done;
-    std::vector<const sth::Plan*> plans(active.size());
     int rc = STH_OK;
-    for (int64_t k = 0; k < K; ++k) {
-      for (size_t i = 0; i < active.size(); ++i) plans[i] = &active[i]->plan[active[i]->cur];
-      if (serial) {
-        for (const sth::Plan* p : plans)
-          if ((rc = sth::run_plan(*p, stream))) return rc;
-      } else if ((rc = launch_batched(ctx, plans.data(), static_cast<int32_t>(plans.size()), false, stream))) {
-        return rc;
-      }
-      for (auto& q : active) {
+    if (serial) {  // the plain per-request path: one request, one launch per step
+      if (active.empty()) return STH_OK;
+      Req& q = *active.front();
+      for (; q.done < q.r.steps; ++q.done, q.cur ^= 1, *launches += 1)
+        if ((rc = sth::run_plan(q.plan[q.cur], stream))) return rc;
+      return STH_OK;
+    }
+    std::vector<Req*> shared, alone;
+    for (auto& q : active) (solo(*q) ? alone : shared).push_back(q.get());
+    int64_t K = R;
+    for (Req* q : shared) K = std::min(K, q->r.steps - q->done);
+    if (shared.empty())
+      for (Req* q : alone) K = std::min(K, q->r.steps - q->done);
+    // the shared batch: one launch per step over every shared request
+    std::vector<const sth::Plan*> plans(shared.size());
+    for (int64_t k = 0; k < K && !shared.empty(); ++k) {
+      for (size_t i = 0; i < shared.size(); ++i) plans[i] = &shared[i]->plan[shared[i]->cur];
+      if ((rc = launch_batched(ctx, plans.data(), static_cast<int32_t>(plans.size()), false, stream))) return rc;
+      for (Req* q : shared) {
         q->cur ^= 1;
         q->done += 1;
       }
       *launches += 1;
+    }
+    // solo requests: up to 4 rounds' worth of steps, never leaving a single
+    // step for the next round (the last two steps are single steps)
+    for (Req* q : alone) {
+      const int64_t rem = q->r.steps - q->done;
+      int64_t n = std::min(rem, std::max(K, 4 * R));
+      if (rem - n == 1) n = n > 1 ? n - 1 : rem;
+      if ((rc = advance_solo(*q, n, launches))) return rc;
     }
     return STH_OK;
   }

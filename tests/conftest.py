@@ -6,6 +6,7 @@ logic and check that libstencils_hip.so exports the ABI.  GPU tests call the
 HIP path through the C ABI and compare it with the oracle.
 """
 import json
+import os
 import sys
 from pathlib import Path
 
@@ -16,6 +17,10 @@ REPO = Path(__file__).resolve().parent.parent
 sys.path.insert(0, str(REPO / "stencils.jl_amd"))
 sys.path.insert(0, str(REPO / "oracle"))
 sys.path.insert(0, str(REPO / "tests"))
+
+# Two-step 3-D launches run by default only on large arrays; tests run them
+# on every eligible array so small shapes cover their boundary cases.
+os.environ.setdefault("STH_FUSE3D_MIN_CELLS", "0")
 
 GOLDEN = json.loads((REPO / "tests" / "golden" / "reference_goldens.json").read_text())
 

@@ -246,6 +246,51 @@ def test_heat_3d_iterate(gpu, oracle, dt):
         np.testing.assert_array_equal(np.asarray(G), np.asarray(O))
 
 
+@pytest.mark.parametrize("pad", ["cond", "halo_out", "halo_in"])
+@pytest.mark.parametrize("bname", ["remove", "remove7", "wrap"])
+@pytest.mark.parametrize("shape", [(70, 17, 35), (3, 3, 3), (130, 40, 33)])
+def test_fused_3d_iterate(gpu, oracle, shape, bname, pad):
+    """Iterations of 3-D radius-1 stencils advance two steps per launch in the
+    steady state; both buffers (halos included) must equal the step-by-step
+    oracle for step counts around the schedule's boundaries."""
+    if pad == "halo_in":  # the halo lies inside the array: keep the same interior
+        shape = tuple(n + 2 for n in shape)
+    for dt, st, f in ((np.float32, sh.VonNeumann(1, 3), sh.Laplacian(0.15)),
+                      (np.float64, sh.VonNeumann(1, 3), sh.Laplacian(0.1)),
+                      (np.float32, sh.Moore(1, 3), sh.mean),
+                      (np.float64, sh.Moore(1, 3), sh.maximum)):
+        A = _np_input(shape, dt, sum(shape))
+        bc = BOUNDS[bname](dt)
+        for steps in (6, 7, 10, 13):
+            G = sh.SwitchingStencilArray(A.copy(order="F"), st, boundary=bc, padding=PADS[pad])
+            O = sh.SwitchingStencilArray(A.copy(order="F"), st, boundary=bc, padding=PADS[pad])
+            G = sh.mapstencil_steps(f, G, steps, graph=steps % 2 == 0)
+            O = oracle.mapstencil_steps(f, O, steps)
+            np.testing.assert_array_equal(G.source, O.source, err_msg=f"{dt} {st} {steps}")
+            np.testing.assert_array_equal(G.dest, O.dest, err_msg=f"{dt} {st} {steps}")
+
+
+@pytest.mark.parametrize("tdt", ["bfloat16", "float16"])
+def test_fused_3d_iterate_storage_types(gpu, tdt):
+    """16-bit storage: the fused launches round the intermediate step to the
+    storage type, so results equal the unfused step-by-step path bit for bit."""
+    import torch
+    dev = torch.device("cuda")
+    dt = getattr(torch, tdt)
+    shape = (96, 37, 41)
+    A = sh.colmajor_empty(shape, dt, dev)
+    A.copy_(torch.rand(shape, device=dev).to(dt))
+    f = sh.Laplacian(0.15) if tdt == "bfloat16" else sh.mean
+    out = []
+    for flags in (0, sh._abi.NO_FUSE):
+        src = sh.colmajor_empty(shape, dt, dev)
+        src.copy_(A)
+        S = sh.SwitchingStencilArray(src, sh.VonNeumann(1, 3), boundary=sh.Wrap())
+        S = sh.mapstencil_steps(f, S, 17, flags=flags)
+        out.append((S.source.clone(), S.dest.clone()))
+    assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1])
+
+
 # ------------------------------------------------------------ errors
 
 def test_argument_errors(gpu):

@@ -133,6 +133,32 @@ def test_budget_admission(gpu):
         assert torch.equal(c.result, _alone(f, S, steps))
 
 
+def test_large_requests_two_step_launches(gpu):
+    """Requests that fill the device run their own two-step launches mixed
+    with batched small ones; results (halo included) equal the step-by-step
+    path alone."""
+    import torch
+    dev = torch.device("cuda")
+    reqs = []
+    for i, (shape, bc, pad, steps) in enumerate((((256, 256, 128), sh.Wrap(), sh.Conditional(), 37),
+                                                 ((250, 260, 130), sh.Remove(np.float32(0.25)), sh.Halo(), 30),
+                                                 ((256, 256, 128), sh.Wrap(), sh.Conditional(), 2),
+                                                 ((256, 256, 128), sh.Wrap(), sh.Conditional(), 5))):
+        A = sh.colmajor_empty(shape, torch.float32, dev)
+        A.copy_(torch.rand(shape, device=dev))
+        reqs.append((sh.Laplacian(0.15), sh.SwitchingStencilArray(A, sh.VonNeumann(1, 3), boundary=bc,
+                                                                  padding=pad), steps, 0.0))
+    reqs += _mixed_requests(3, 6)
+    done, tags, stats = _serve(reqs, hbm_budget=1 << 30, round_steps=4)
+    assert len(done) == len(reqs)
+    for c in done:
+        f, S, steps = tags[c.tag]
+        ref = _alone(f, S, steps)
+        assert torch.equal(_interior(S, c.result), _interior(S, ref)), (S, steps)
+        if steps >= 2 and S.sth_dtype() == sh._abi.F32 and S.size()[0] >= 250:
+            assert torch.equal(c.result, ref), ("halo", S, steps)
+
+
 def test_request_errors(gpu):
     import torch
     dev = torch.device("cuda")
