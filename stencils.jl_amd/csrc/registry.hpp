@@ -60,6 +60,7 @@ struct Fused2Entry {
   int ndim;
   const int8_t* offs;
   LaunchFn fn;
+  int rows;  // rows per wave of the z-marching kernel
 };
 std::vector<Fused2Entry>& fused2_registry();
 

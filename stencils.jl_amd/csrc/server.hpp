@@ -152,9 +152,14 @@ struct sth_server {
         if (rc == STH_OK) sth::apply_variant(q.plan[c], cands[idx]);
       }
       if (rc) return rc;
-      hipError_t e = hipMemcpy2DAsync(arena + o0, q.pitch, q.r.state.data, q.row_bytes, q.row_bytes,
-                                      static_cast<size_t>(q.rows), hipMemcpyDefault, stream);
-      if (e != hipSuccess) return sth::fail(STH_EDEVICE, "copy-in: %s", hipGetErrorString(e));
+      // both buffers start as the state, as a SwitchingStencilArray's source
+      // and dest do (src/array.jl:593-598)
+      for (const size_t o : {o0, o1}) {
+        const hipError_t ce = hipMemcpy2DAsync(arena + o, q.pitch, q.r.state.data, q.row_bytes, q.row_bytes,
+                                               static_cast<size_t>(q.rows), hipMemcpyDefault, stream);
+        if (ce != hipSuccess) return sth::fail(STH_EDEVICE, "copy-in: %s", hipGetErrorString(ce));
+      }
+      hipError_t e;
       e = hipEventCreate(&q.fin);
       if (e != hipSuccess) return sth::fail(STH_EDEVICE, "hipEventCreate: %s", hipGetErrorString(e));
       q.admitted = t;
