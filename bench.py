@@ -37,14 +37,14 @@ def device_switching(A: np.ndarray, sh, torch):
     return sh.SwitchingStencilArray(t, sh.Moore(1), boundary=sh.Wrap(), padding=sh.Halo())
 
 
-def time_steps(sh, torch, S, rule, steps: int, graph: bool):
+def time_steps(sh, torch, S, rule, steps: int, graph: bool, flags: int = 0):
     """Wall time and HIP-event time of `steps` generations (inputs resident)."""
     stream = torch.cuda.current_stream()
     start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     start.record(stream)
-    S = sh.mapstencil_steps(rule, S, steps, graph=graph, flags=sh._abi.ASYNC)
+    S = sh.mapstencil_steps(rule, S, steps, graph=graph, flags=sh._abi.ASYNC | flags)
     end.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -340,7 +340,56 @@ def bench_served(sh, torch, n: int = 120, budget: int = 2 << 30):
             res["served_vs_serial"] = {"throughput": summ["cell_steps_per_s"] / ser["cell_steps_per_s"],
                                        "p50_latency": ser["p50_s"] / summ["p50_s"],
                                        "p99_latency": ser["p99_s"] / summ["p99_s"]}
+            # capacity: the same requests all arriving at t = 0, so both paths
+            # are saturated and throughput measures the device, not the arrivals
+            burst = [(S, steps, 0.0) for S, steps, _ in reqs]
+            _, _, cap = run(burst, False)
+            _, _, cap_ser = run(burst, True)
+            res["f32_burst_capacity"] = {
+                "served_cell_steps_per_s": cap["cell_steps_per_s"],
+                "serial_plain_cell_steps_per_s": cap_ser["cell_steps_per_s"],
+                "throughput_ratio": cap["cell_steps_per_s"] / cap_ser["cell_steps_per_s"],
+                "served_span_s": cap["span_s"], "serial_span_s": cap_ser["span_s"]}
         del reqs, done, tags
+        torch.cuda.empty_cache()
+    return res
+
+
+def bench_heat_iterate(sh, torch, n: int = 512, steps: int = 200):
+    """3-D heat iteration (VonNeumann(1,3) Laplacian(0.15), Wrap, n^3) as
+    `steps` hipGraph-replayed steps: two steps per launch (zmarch2_kernel)
+    against one streaming step per launch (STH_NO_FUSE).  The two-step
+    kernel's roofline counts both steps' algorithmic bytes per launch."""
+    f = sh.Laplacian(0.15)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    stream = torch.cuda.current_stream()
+    res = {"workload": f"heat {n}^3 VonNeumann(1,3) Laplacian(0.15) Wrap, {steps} steps, hipGraph"}
+    for name, tdt in (("f32", torch.float32), ("bf16", torch.bfloat16), ("f64", torch.float64)):
+        A = sh.colmajor_empty((n,) * 3, tdt, dev)
+        A.copy_(torch.rand((n,) * 3, device=dev).to(tdt))
+        S = sh.SwitchingStencilArray(A, sh.VonNeumann(1, 3), boundary=sh.Wrap())
+        t = {}
+        for key, fl in (("two_step", 0), ("one_step", sh._abi.NO_FUSE)):
+            S = sh.mapstencil_steps(f, S, 8, flags=sh._abi.ASYNC | fl)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            e0.record(stream)
+            S = sh.mapstencil_steps(f, S, steps, flags=sh._abi.ASYNC | fl)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            t[key] = e0.elapsed_time(e1) / steps * 1e3  # us per step
+        es = A.element_size()
+        cells = n ** 3
+        launch_us = 2 * t["two_step"]
+        res[name] = {"two_step_us_per_step": t["two_step"], "one_step_us_per_step": t["one_step"],
+                     "speedup": t["one_step"] / t["two_step"],
+                     "cell_steps_per_s": cells / (t["two_step"] * 1e-6),
+                     "two_step_roofline": {"bytes_per_launch": 2 * 2 * es * cells, "avg_launch_us": launch_us,
+                                           "achieved_gbs": 2 * 2 * es * cells / (launch_us * 1e-6) / 1e9},
+                     "one_step_achieved_gbs": 2 * es * cells / (t["one_step"] * 1e-6) / 1e9}
+        if name == "f32" and n == 512:
+            res[name]["two_step_roofline"]["traffic"] = profiled_traffic("zmarch2_kernel_heat_512.md")
+        del A, S
         torch.cuda.empty_cache()
     return res
 
@@ -355,6 +404,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the secondary workloads (mean blur, Gaussian)")
+    ap.add_argument("--only", choices=["heat3d", "served", "batched"],
+                    help="run one secondary workload alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
@@ -366,6 +417,12 @@ def main():
     import stencils_hip as sh
     sh.load_library()
     torch.cuda.set_device(int(os.environ.get("LOCAL_RANK", 0)))
+    if args.only:
+        run_one = {"heat3d": lambda: bench_heat_iterate(sh, torch, steps=args.steps),
+                   "served": lambda: bench_served(sh, torch),
+                   "batched": lambda: bench_batched(sh, torch)}[args.only]
+        print(json.dumps({args.only: run_one()}), flush=True)
+        return
 
     n = args.n
     A = life_grid(n)
@@ -399,9 +456,9 @@ def main():
     copy_gbs = 2 * cells / (e0.elapsed_time(e1) / 1e3 / 50) / 1e9
     del src_t, dst_t
 
-    # the same steps as plain per-step launches (no graph)
-    S, _, _ = time_steps(sh, torch, S, rule, 10, graph=False)
-    S, wall_plain, _ = time_steps(sh, torch, S, rule, args.steps, graph=False)
+    # the same steps as plain per-step launches (no graph, one generation each)
+    S, _, _ = time_steps(sh, torch, S, rule, 10, graph=False, flags=sh._abi.NO_FUSE)
+    S, wall_plain, _ = time_steps(sh, torch, S, rule, args.steps, graph=False, flags=sh._abi.NO_FUSE)
 
     out = {
         "metric": METRIC,
@@ -444,6 +501,7 @@ def main():
         out["configs"] = {"mean_blur_f64": bench_mean_blur(sh, torch, not args.no_cpu_baseline),
                           "gaussian_7x7": bench_gaussian(sh, torch),
                           "batched_uneven": bench_batched(sh, torch),
+                          "heat_3d_iterate": bench_heat_iterate(sh, torch),
                           "served_heat_3d": bench_served(sh, torch)}
     print(json.dumps(out), flush=True)
 

@@ -14,10 +14,10 @@
 //   3. advance -- K = min(round_steps, fewest remaining steps) steps; each
 //                 step is one batched launch over the active requests
 //                 (requests join and leave only between rounds), buffers
-//                 ping-pong per request.  Requests that fill the device by
-//                 themselves and have a two-step kernel run their own
-//                 launches instead, two steps each, up to 4 x round_steps
-//                 steps per round.  Requests that reach their step count get
+//                 ping-pong per request.  Requests of 128^3 cells or more
+//                 that have a two-step kernel run their own launches
+//                 instead, two steps each, up to 4 x round_steps steps per
+//                 round.  Requests that reach their step count get
 //                 their state copied to `result` and an event recorded.
 // Launches are stream-ordered, so a freed buffer is reused only by work
 // enqueued after the copy that last read it.  The host runs at most
@@ -212,11 +212,12 @@ struct sth_server {
   }
 
   // --- 3. one round (called without `mu`: only the scheduler touches `active`)
-  // A request that runs as launches of its own (it fills the device by
-  // itself) and has a two-step kernel: it advances two steps per launch and
-  // more steps per round than the shared batch.
+  // A request with a two-step kernel and at least 2^21 cells (128^3) runs as
+  // launches of its own: two steps per launch, and more steps per round than
+  // the shared batch.  Below that size batching the steps pays more.
+  static constexpr int64_t kSoloCells = int64_t(1) << 21;
   bool solo(const Req& q) const {
-    return q.plan[0].fn_fused2 && q.plan[1].fn_fused2 && sth::runs_alone(q.plan[0], ctx->cus);
+    return q.plan[0].fn_fused2 && q.plan[1].fn_fused2 && q.cells >= kSoloCells;
   }
 
   // `n` steps of a solo request.  Fused launches leave the buffers holding
