@@ -48,7 +48,17 @@ struct sth_server {
     int cur = 0;
     double admitted = 0;
     hipEvent_t fin = nullptr;
+    int lane = -1;              // solo requests: index of their stream
+    hipEvent_t join = nullptr;  // orders a solo stream after the main one and back
   };
+
+  // Solo requests run on a few streams of their own (default priority), so
+  // requests too small to fill the device alone overlap each other and the
+  // shared batch.  Each is pinned to one stream; events order its work after
+  // its copy-in and its copy-out after its work.
+  static constexpr int kSoloStreams = 4;
+  hipStream_t solo_streams[kSoloStreams] = {};
+  int next_lane = 0;
 
   sth_ctx* ctx = nullptr;
   hipStream_t stream = nullptr;
@@ -162,6 +172,13 @@ struct sth_server {
       hipError_t e;
       e = hipEventCreate(&q.fin);
       if (e != hipSuccess) return sth::fail(STH_EDEVICE, "hipEventCreate: %s", hipGetErrorString(e));
+      if (solo(q)) {
+        q.lane = next_lane++ % kSoloStreams;
+        e = hipEventCreateWithFlags(&q.join, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventRecord(q.join, stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(solo_streams[q.lane], q.join, 0);
+        if (e != hipSuccess) return sth::fail(STH_EDEVICE, "solo stream join: %s", hipGetErrorString(e));
+      }
       q.admitted = t;
       active.push_back(std::move(pending.front()));
       pending.pop_front();
@@ -195,6 +212,7 @@ struct sth_server {
       release(q.off[0], q.bytes_one);
       release(q.off[1], q.bytes_one);
       (void)hipEventDestroy(q.fin);
+      if (q.join) (void)hipEventDestroy(q.join);
       retiring.erase(retiring.begin() + static_cast<std::ptrdiff_t>(i));
       done_cv.notify_all();
     }
@@ -203,6 +221,11 @@ struct sth_server {
 
   int finish(Req& q) {
     const int c = q.cur;
+    if (q.lane >= 0) {  // the copy-out waits for the request's own stream
+      hipError_t je = hipEventRecord(q.join, solo_streams[q.lane]);
+      if (je == hipSuccess) je = hipStreamWaitEvent(stream, q.join, 0);
+      if (je != hipSuccess) return sth::fail(STH_EDEVICE, "solo stream join: %s", hipGetErrorString(je));
+    }
     hipError_t e = hipMemcpy2DAsync(q.r.result, q.row_bytes, arena + q.off[c], q.pitch, q.row_bytes,
                                     static_cast<size_t>(q.rows), hipMemcpyDefault, stream);
     if (e != hipSuccess) return sth::fail(STH_EDEVICE, "copy-out: %s", hipGetErrorString(e));
@@ -215,9 +238,16 @@ struct sth_server {
   // A request with a two-step kernel and at least 2^21 cells (128^3) runs as
   // launches of its own: two steps per launch, and more steps per round than
   // the shared batch.  Below that size batching the steps pays more.
-  static constexpr int64_t kSoloCells = int64_t(1) << 21;
+  // (STH_SERVER_SOLO_CELLS overrides the size)
+  static int64_t solo_cells() {
+    static const int64_t v = [] {
+      const char* e = std::getenv("STH_SERVER_SOLO_CELLS");
+      return e ? std::atoll(e) : (int64_t(1) << 21);
+    }();
+    return v;
+  }
   bool solo(const Req& q) const {
-    return q.plan[0].fn_fused2 && q.plan[1].fn_fused2 && q.cells >= kSoloCells;
+    return q.plan[0].fn_fused2 && q.plan[1].fn_fused2 && q.cells >= solo_cells();
   }
 
   // `n` steps of a solo request.  Fused launches leave the buffers holding
@@ -226,16 +256,17 @@ struct sth_server {
   // a request's last two steps are always single steps, so its final state,
   // halo included, is the step-by-step one.
   int advance_solo(Req& q, int64_t n, int64_t* launches) {
+    const hipStream_t st = solo_streams[q.lane];
     const int64_t rem = q.r.steps - q.done;
     const int64_t fused = n == rem ? std::max<int64_t>(0, (n - 2) / 2) : n / 2;
     for (int64_t i = 0; i < fused; ++i) {
-      if (int rc = sth::run_fused2(q.plan[q.cur], stream)) return rc;
+      if (int rc = sth::run_fused2(q.plan[q.cur], st)) return rc;
       q.cur ^= 1;
       q.done += 2;
       *launches += 1;
     }
     for (int64_t i = 2 * fused; i < n; ++i) {
-      if (int rc = sth::run_plan(q.plan[q.cur], stream)) return rc;
+      if (int rc = sth::run_plan(q.plan[q.cur], st)) return rc;
       q.cur ^= 1;
       q.done += 1;
       *launches += 1;
@@ -255,7 +286,7 @@ struct sth_server {
       return STH_OK;
     }
     std::vector<Req*> shared, alone;
-    for (auto& q : active) (solo(*q) ? alone : shared).push_back(q.get());
+    for (auto& q : active) (q->lane >= 0 ? alone : shared).push_back(q.get());
     int64_t K = R;
     for (Req* q : shared) K = std::min(K, q->r.steps - q->done);
     if (shared.empty())
@@ -370,9 +401,14 @@ int sth_server_create(sth_ctx* ctx, const sth_server_config* cfg, void* stream, 
   if (e != hipSuccess) return fail(STH_ENOMEM, "arena of %zu bytes: %s", bytes, hipGetErrorString(e));
   srv->free_blocks[0] = bytes;
   e = hipEventCreate(&srv->ev0);
+  for (int i = 0; i < sth_server::kSoloStreams && e == hipSuccess; ++i)
+    e = hipStreamCreateWithFlags(&srv->solo_streams[i], hipStreamNonBlocking);
   if (e != hipSuccess) {
+    for (hipStream_t x : srv->solo_streams)
+      if (x) (void)hipStreamDestroy(x);
+    if (srv->ev0) (void)hipEventDestroy(srv->ev0);
     (void)hipFree(srv->arena);
-    return fail(STH_EDEVICE, "hipEventCreate: %s", hipGetErrorString(e));
+    return fail(STH_EDEVICE, "server events and streams: %s", hipGetErrorString(e));
   }
   *out = srv.release();
   return STH_OK;
@@ -509,9 +545,16 @@ int sth_server_destroy(sth_server* srv) {
   }
   if (srv->thr.joinable()) srv->thr.join();
   (void)hipStreamSynchronize(srv->stream);
+  for (hipStream_t x : srv->solo_streams)
+    if (x) {
+      (void)hipStreamSynchronize(x);
+      (void)hipStreamDestroy(x);
+    }
   for (auto* v : {&srv->active, &srv->retiring})
-    for (auto& q : *v)
+    for (auto& q : *v) {
       if (q->fin) (void)hipEventDestroy(q->fin);
+      if (q->join) (void)hipEventDestroy(q->join);
+    }
   if (srv->ev0) (void)hipEventDestroy(srv->ev0);
   if (srv->arena) (void)hipFree(srv->arena);
   delete srv;
