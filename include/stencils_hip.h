@@ -187,9 +187,10 @@ int sth_map(sth_ctx* ctx, const sth_array* src, const sth_array* dst,
  * (src/gatherstencil.jl:77-83 + switch, src/array.jl:610-611), ping-ponging
  * buffers a (source) and b (dest) on device.  On return *current is 0 if
  * `a` holds the newest state, 1 if `b` does (the switched wrapper's source).
- * With STH_GRAPH the steps are captured once into hipGraphs and replayed;
- * Life-like rules then advance two generations per launch in the steady
- * steps (temporal blocking, STH_NO_FUSE turns it off) and finish with single
+ * With STH_GRAPH the steps are captured once into hipGraphs and replayed.
+ * Life rules (2-D) and 3-D radius-1 catalogue stencils under Wrap or Remove
+ * (arrays of 128^3 cells or more) advance two steps per launch in the steady
+ * steps (temporal blocking; STH_NO_FUSE turns it off) and finish with single
  * steps, so both buffers end exactly as the step-by-step reference leaves them.
  */
 int sth_iterate(sth_ctx* ctx, const sth_array* a, const sth_array* b,
@@ -237,10 +238,11 @@ typedef struct sth_job {
 
 /*
  * Many independent gatherstencil!(f, dest, source) calls -- mixed sizes,
- * shapes and reductions -- in one persistent launch per element type
- * (workgroups pull tiles from a device-side counter).  Halo jobs get their
- * update_boundary! refresh first; jobs outside the batched catalogue run as
- * individual launches inside the call.  Each result is identical to an
+ * shapes and reductions -- in one launch per (element type, shape,
+ * reduction): every job's tiles form one grid, and each workgroup finds its
+ * job from its tile index.  Halo jobs get their update_boundary! refresh
+ * first; jobs outside the batched catalogue, and jobs large enough to fill
+ * the device alone, run as individual launches inside the call.  Each result is identical to an
  * sth_map call on that job alone.  No counterpart in the reference, which
  * runs one KernelAbstractions launch per mapstencil call
  * (src/gatherstencil.jl:97-100); callers: the Julia shim's batched
@@ -258,9 +260,12 @@ int sth_map_batch(sth_ctx* ctx, int32_t njobs, const sth_job* jobs, uint32_t fla
  * bytes of device memory, admits requests in arrival order as soon as their
  * arrival time has passed and their buffers fit (and fewer than `max_active`
  * run), and advances every admitted request together: each step of a round
- * is one persistent launch over all of them (sth_map_batch's kernel), so
- * requests that arrive while others run join at the next round.  Finished
- * states are copied to the request's `result` and their buffers freed.
+ * is one batched launch over all of them (sth_map_batch's kernels), so
+ * requests that arrive while others run join at the next round.  Requests
+ * of at least 128^3 cells with a two-step kernel (3-D radius-1 stencils,
+ * Wrap or Remove) run their own two-step launches on their own streams
+ * instead.  Finished states (halo included) are copied to the request's
+ * `result` and their buffers freed.
  * STH_SERVER_SERIAL instead serves one request at a time with one plain
  * launch per step (the unbatched path, for comparison).
  * No counterpart in the reference; callers: a serving front end (the

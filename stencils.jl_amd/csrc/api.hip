@@ -666,8 +666,9 @@ int tune_pick(sth_ctx* ctx, const Plan& pl, const std::vector<Variant>& c, const
 }
 
 // One step of many independent calls: calls with a batched variant go into
-// one persistent launch per element type (after their halo refresh), the
-// rest run as single launches first.  `serial_only` runs every call alone.
+// one launch per (element type, shape, reduction) after their halo refresh;
+// the rest, and calls that fill the device alone, run as single launches
+// first.  `serial_only` runs every call alone.
 int launch_batched(sth_ctx* ctx, const Plan* const* plans, int32_t n, bool serial_only, hipStream_t s) {
   int rc = STH_OK;
   std::map<int64_t, std::vector<BatchJob>> groups;

@@ -341,7 +341,7 @@ class BatchPlan:
 def mapstencil_batch(calls, serial: bool = False, flags: int = 0):
     """Many gatherstencil!(f, dest, source) calls in one library call
     (sth_map_batch): jobs of mixed sizes, shapes and reductions share one
-    persistent launch per element type.  `calls` is a sequence of
+    launch per (element type, shape, reduction).  `calls` is a sequence of
     (f, dest, source) with device tensors; `serial` runs one launch per job
     instead (the unbatched path).  Results equal the one-call-per-job path."""
     BatchPlan(calls).run(serial=serial, flags=flags)
