@@ -547,12 +547,18 @@ bool runs_alone(const Plan& pl, int cus) {
 struct BatchSlot {
   void* host = nullptr;
   void* dev = nullptr;
-  size_t bytes = 0;
+  size_t bytes = 0;     // capacity
+  size_t content = 0;   // bytes of the table the device copy holds
+  uint64_t last_use = 0;
   hipEvent_t done = nullptr;
   bool used = false;
 };
 
-constexpr int kBatchSlots = 8;  // job tables in flight (a launch reuses a slot once it has finished)
+// Job tables kept on the device.  A call whose table equals one already
+// there launches on it without a copy (the serving loop alternates between
+// two tables per group, one per step parity); otherwise the least recently
+// used slot is overwritten, once its last launch has finished with it.
+constexpr int kBatchSlots = 32;
 
 struct sth_ctx {
   int device = 0;
@@ -561,7 +567,7 @@ struct sth_ctx {
   std::mutex mu;
   std::map<std::string, hipGraphExec_t> graphs;
   BatchSlot slots[kBatchSlots];
-  unsigned next_slot = 0;
+  uint64_t slot_clock = 0;
   // autotuner: shape family + size bucket -> chosen variant name
   std::map<std::string, std::string> tuned;
   bool autotune = true;    // STH_AUTOTUNE=0 disables timing (defaults are used)
@@ -721,9 +727,19 @@ int launch_batched(sth_ctx* ctx, const Plan* const* plans, int32_t n, bool seria
       return fail(STH_ENOTSUP, "batch of %lld tiles is too large", static_cast<long long>(total));
     const size_t bytes = bj.size() * sizeof(BatchJob);
     std::lock_guard<std::mutex> lk(ctx->mu);
-    BatchSlot& sl = ctx->slots[ctx->next_slot++ % kBatchSlots];
-    if (sl.used) STH_HIP(hipEventSynchronize(sl.done));  // its last launch has finished with it
-    if (sl.bytes < bytes) {
+    BatchSlot* hit = nullptr;
+    for (BatchSlot& c : ctx->slots)
+      if (c.used && c.content == bytes && std::memcmp(c.host, bj.data(), bytes) == 0) {
+        hit = &c;
+        break;
+      }
+    BatchSlot* lru = &ctx->slots[0];
+    for (BatchSlot& c : ctx->slots)
+      if (c.last_use < lru->last_use) lru = &c;
+    BatchSlot& sl = hit ? *hit : *lru;
+    sl.last_use = ++ctx->slot_clock;
+    if (!hit && sl.used) STH_HIP(hipEventSynchronize(sl.done));  // its last launch has finished with it
+    if (!hit && sl.bytes < bytes) {
       if (sl.host) STH_HIP(hipHostFree(sl.host));
       if (sl.dev) STH_HIP(hipFree(sl.dev));
       sl.host = sl.dev = nullptr;
@@ -734,8 +750,15 @@ int launch_batched(sth_ctx* ctx, const Plan* const* plans, int32_t n, bool seria
       sl.bytes = cap;
     }
     if (!sl.done) STH_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
-    std::memcpy(sl.host, bj.data(), bytes);
-    STH_HIP(hipMemcpyAsync(sl.dev, sl.host, bytes, hipMemcpyHostToDevice, s));
+    if (!hit) {
+      std::memcpy(sl.host, bj.data(), bytes);
+      STH_HIP(hipMemcpyAsync(sl.dev, sl.host, bytes, hipMemcpyHostToDevice, s));
+      sl.content = bytes;
+    } else {
+      // after the slot's upload and every launch on it so far, whatever stream
+      // they ran on (so an overwrite that waits for `done` waits for all)
+      STH_HIP(hipStreamWaitEvent(s, sl.done, 0));
+    }
     BatchParams bp;
     bp.jobs = static_cast<const BatchJob*>(sl.dev);
     bp.njobs = static_cast<int32_t>(bj.size());

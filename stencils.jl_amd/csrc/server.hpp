@@ -20,9 +20,9 @@
 //                 round.  Requests that reach their step count get
 //                 their state copied to `result` and an event recorded.
 // Launches are stream-ordered, so a freed buffer is reused only by work
-// enqueued after the copy that last read it.  The host runs at most
-// kBatchSlots launches ahead of the device (the batched launcher's job-table
-// ring waits for the oldest), which bounds the admission delay.
+// enqueued after the copy that last read it.  The host runs at most two
+// rounds ahead of the device (events per round and stream), which bounds the
+// admission delay.
 #pragma once
 
 #include <chrono>
@@ -59,6 +59,13 @@ struct sth_server {
   static constexpr int kSoloStreams = 4;
   hipStream_t solo_streams[kSoloStreams] = {};
   int next_lane = 0;
+
+  // The scheduler enqueues at most kLookahead rounds ahead of the device:
+  // a round starts once the round kLookahead before it has finished on
+  // every stream, so a newly admitted request waits behind little queued work.
+  static constexpr int kLookahead = 2;
+  hipEvent_t marks[kLookahead][1 + kSoloStreams] = {};
+  int64_t round_no = 0;
 
   sth_ctx* ctx = nullptr;
   hipStream_t stream = nullptr;
@@ -275,6 +282,20 @@ struct sth_server {
   }
 
   int advance(int64_t* launches) {
+    hipEvent_t* mark = marks[round_no % kLookahead];
+    if (round_no >= kLookahead)
+      for (int i = 0; i <= kSoloStreams; ++i)
+        if (hipEventSynchronize(mark[i]) != hipSuccess) return sth::fail(STH_EDEVICE, "round mark wait failed");
+    const int rc = advance_round(launches);
+    if (rc) return rc;
+    for (int i = 0; i <= kSoloStreams; ++i)
+      if (hipEventRecord(mark[i], i ? solo_streams[i - 1] : stream) != hipSuccess)
+        return sth::fail(STH_EDEVICE, "round mark record failed");
+    ++round_no;
+    return STH_OK;
+  }
+
+  int advance_round(int64_t* launches) {
     const int64_t R = cfg.round_steps > 0 ? cfg.round_steps : 8;
     const bool serial = (cfg.flags & STH_SERVER_SERIAL) != 0;
     int rc = STH_OK;
@@ -403,9 +424,15 @@ int sth_server_create(sth_ctx* ctx, const sth_server_config* cfg, void* stream, 
   e = hipEventCreate(&srv->ev0);
   for (int i = 0; i < sth_server::kSoloStreams && e == hipSuccess; ++i)
     e = hipStreamCreateWithFlags(&srv->solo_streams[i], hipStreamNonBlocking);
+  for (auto& row : srv->marks)
+    for (hipEvent_t& m : row)
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&m, hipEventDisableTiming);
   if (e != hipSuccess) {
     for (hipStream_t x : srv->solo_streams)
       if (x) (void)hipStreamDestroy(x);
+    for (auto& row : srv->marks)
+      for (hipEvent_t m : row)
+        if (m) (void)hipEventDestroy(m);
     if (srv->ev0) (void)hipEventDestroy(srv->ev0);
     (void)hipFree(srv->arena);
     return fail(STH_EDEVICE, "server events and streams: %s", hipGetErrorString(e));
@@ -555,6 +582,9 @@ int sth_server_destroy(sth_server* srv) {
       if (q->fin) (void)hipEventDestroy(q->fin);
       if (q->join) (void)hipEventDestroy(q->join);
     }
+  for (auto& row : srv->marks)
+    for (hipEvent_t m : row)
+      if (m) (void)hipEventDestroy(m);
   if (srv->ev0) (void)hipEventDestroy(srv->ev0);
   if (srv->arena) (void)hipFree(srv->arena);
   delete srv;
