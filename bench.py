@@ -404,7 +404,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the secondary workloads (mean blur, Gaussian)")
-    ap.add_argument("--only", choices=["heat3d", "served", "batched"],
+    ap.add_argument("--only", choices=["heat3d", "served", "batched", "gaussian", "meanblur"],
                     help="run one secondary workload alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -420,7 +420,9 @@ def main():
     if args.only:
         run_one = {"heat3d": lambda: bench_heat_iterate(sh, torch, steps=args.steps),
                    "served": lambda: bench_served(sh, torch),
-                   "batched": lambda: bench_batched(sh, torch)}[args.only]
+                   "batched": lambda: bench_batched(sh, torch),
+                   "gaussian": lambda: bench_gaussian(sh, torch),
+                   "meanblur": lambda: bench_mean_blur(sh, torch, False)}[args.only]
         print(json.dumps({args.only: run_one()}), flush=True)
         return
 

@@ -43,6 +43,37 @@ __device__ __forceinline__ void store_vec(Raw* p, const RawVec<Raw, V>& v) {
   __builtin_memcpy(p, &v, sizeof(v));
 }
 
+// The same vectors at their natural alignment (V * sizeof(Raw) bytes): one
+// load or store instruction even for 16-bit elements.
+template <typename Raw, int V>
+struct alignas(V * sizeof(Raw)) AlignedVec {
+  Raw e[V];
+};
+
+template <typename Raw, int V>
+__device__ __forceinline__ RawVec<Raw, V> load_avec(const Raw* p) {
+  const AlignedVec<Raw, V> a = *reinterpret_cast<const AlignedVec<Raw, V>*>(p);
+  RawVec<Raw, V> r;
+#pragma unroll
+  for (int i = 0; i < V; ++i) r.e[i] = a.e[i];
+  return r;
+}
+
+template <typename Raw, int V>
+__device__ __forceinline__ void store_avec(Raw* p, const RawVec<Raw, V>& v) {
+  AlignedVec<Raw, V> a;
+#pragma unroll
+  for (int i = 0; i < V; ++i) a.e[i] = v.e[i];
+  *reinterpret_cast<AlignedVec<Raw, V>*>(p) = a;
+}
+
+// True when `base` and every row and plane start are multiples of `bytes`.
+__device__ __forceinline__ bool rows_aligned(const void* base, int64_t s1, int64_t s2, int64_t esize,
+                                             int64_t bytes) {
+  return ((reinterpret_cast<uint64_t>(base) | static_cast<uint64_t>(s1 * esize) |
+           static_cast<uint64_t>(s2 * esize)) % static_cast<uint64_t>(bytes)) == 0;
+}
+
 __device__ __forceinline__ int64_t clamp64(int64_t v, int64_t lo, int64_t hi) {
   return v < lo ? lo : (v > hi ? hi : v);
 }
@@ -94,7 +125,34 @@ __device__ __forceinline__ void stream_body(const P& p, int bx, int by, int bz, 
   const bool rows_direct = wy0 - RY >= -h1 && wy0 + RPW * KY - 1 + RY < n1 + h1 && z0 - RZ >= -h2 &&
                            z0 + RZ < n2 + h2;
   const bool wave_x_direct = wx0 - RX >= -h0 && wx0 + LPR * V + RX <= n0 + h0;
-  if (rows_direct && wave_x_direct) {
+  // 16-bit elements on aligned rows whose neighbouring vectors lie inside the
+  // row: the RX cells on either side come from the neighbouring aligned
+  // vectors, three vector loads per row instead of one vector and 2 RX
+  // single 16-bit loads.  (For 4- and 8-byte elements the single loads are
+  // cheaper than the wider vectors: measured 12% slower on a 7x7 f32 kernel.)
+  const bool vec3 = sizeof(Raw) == 2 && RX <= V && rows_direct && wx0 - V >= -h0 &&
+                    wx0 + (LPR + 1) * V <= n0 + h0 && rows_aligned(sbase, s1, s2, sizeof(Raw), V * sizeof(Raw));
+  if (vec3) {
+#pragma unroll
+    for (int iz = 0; iz < NZ; ++iz) {
+#pragma unroll
+      for (int iy = 0; iy < NY; ++iy) {
+        const Raw* row = sbase + (z0 - RZ + iz) * s2 + (y0 - RY + iy) * s1 + x0;
+        const RawVec<Raw, V> m = load_avec<Raw, V>(row);
+#pragma unroll
+        for (int i = 0; i < V; ++i) v[iz][iy][RX + i] = stage<S>(m.e[i]);
+        if constexpr (RX > 0) {
+          const RawVec<Raw, V> l = load_avec<Raw, V>(row - V);
+          const RawVec<Raw, V> r = load_avec<Raw, V>(row + V);
+#pragma unroll
+          for (int i = 0; i < RX; ++i) {
+            v[iz][iy][i] = stage<S>(l.e[V - RX + i]);
+            v[iz][iy][RX + V + i] = stage<S>(r.e[i]);
+          }
+        }
+      }
+    }
+  } else if (rows_direct && wave_x_direct) {
     // ---- interior wave: straight rows, one vector + 2 RX cells per row
 #pragma unroll
     for (int iz = 0; iz < NZ; ++iz) {
@@ -167,6 +225,7 @@ __device__ __forceinline__ void stream_body(const P& p, int bx, int by, int bz, 
 
   // ---- fold and store
   ORaw* __restrict__ dbase = reinterpret_cast<ORaw*>(p.dst.base) + z0 * p.dst.s2;
+  const bool dst_aligned = rows_aligned(p.dst.base, p.dst.s1, p.dst.s2, sizeof(ORaw), V * sizeof(ORaw));
 #pragma unroll
   for (int j = 0; j < KY; ++j) {
     RawVec<ORaw, V> out;
@@ -183,7 +242,10 @@ __device__ __forceinline__ void stream_body(const P& p, int bx, int by, int bz, 
     if (y < n1) {
       ORaw* drow = dbase + y * p.dst.s1;
       if (x0 + V <= n0) {
-        store_vec<ORaw, V>(drow + x0, out);
+        if (dst_aligned)
+          store_avec<ORaw, V>(drow + x0, out);
+        else
+          store_vec<ORaw, V>(drow + x0, out);
       } else {
 #pragma unroll
         for (int i = 0; i < V; ++i)
