@@ -27,7 +27,8 @@ using Stencils: AbstractStencilArray, StencilArray, SwitchingStencilArray, Stenc
     neighbors, center, kernel
 using AMDGPU: ROCArray
 
-export LifeRule, Laplacian, mapstencil_steps!, sth_available
+export LifeRule, Laplacian, mapstencil_steps!, mapstencil_batch!, sth_available
+export Server, submit!, start!, completions!, drain!, close!
 
 const libsth = get(ENV, "STENCILS_HIP_LIB",
                    joinpath(@__DIR__, "..", "lib", "libstencils_hip.so"))
@@ -77,6 +78,48 @@ struct SthOp
     alpha::Float64
     birth_mask::UInt64
     survive_mask::UInt64
+end
+
+struct SthJob
+    src::SthArray
+    dst::SthArray
+    stencil::SthStencil
+    op::SthOp
+    padval::Ptr{Cvoid}
+    boundary::Int32
+    padding::Int32
+end
+
+struct SthServerConfig
+    hbm_budget::Csize_t
+    max_active::Int32
+    round_steps::Int32
+    flags::UInt32
+    reserved::Int32
+end
+
+struct SthRequest
+    state::SthArray
+    stencil::SthStencil
+    op::SthOp
+    padval::Ptr{Cvoid}
+    boundary::Int32
+    padding::Int32
+    steps::Int64
+    result::Ptr{Cvoid}
+    arrival::Float64
+    tag::UInt64
+end
+
+struct SthCompletion
+    tag::UInt64
+    status::Int32
+    reserved::Int32
+    arrival::Float64
+    admitted::Float64
+    finished::Float64
+    steps::Int64
+    cells::Int64
 end
 
 # ---------------------------------------------------- canonical rule callables
@@ -257,6 +300,143 @@ function _fallback_steps(f, A, steps)
         A = Stencils.gatherstencil!(f, A)
     end
     return A
+end
+
+# ------------------------------------------------------------ batched calls
+"""
+    mapstencil_batch!(calls)
+
+Many `gatherstencil!(f, dest, source)` calls, `calls` a vector of
+`(f, dest::ROCArray, source::StencilArray)` over device arrays, in one
+library call (`sth_map_batch`): one launch per (element type, shape,
+reduction) instead of one per call.  Each `dest` ends exactly as the single
+call leaves it.  When any call is outside the accelerated set, every call
+runs through `gatherstencil!` one by one.
+"""
+function mapstencil_batch!(calls::AbstractVector)
+    keep = Any[]
+    jobs = SthJob[]
+    for (f, dest, src) in calls
+        T = eltype(src)
+        op = haskey(DTYPES, T) ? _op(f, T) : nothing
+        pv = _padval(boundary(src), T)
+        if op === nothing || pv === nothing
+            foreach(c -> Stencils.gatherstencil!(c...), calls)
+            return calls
+        end
+        offs, kind, r, n, l = _stencil_desc(stencil(src))
+        w = _weights(stencil(src), T)
+        push!(keep, offs, w, pv)
+        sd = SthStencil(kind, r, n, l, pointer(offs),
+                        w === nothing ? C_NULL : Ptr{Cvoid}(pointer(w)),
+                        w === nothing ? Int32(-1) : DTYPES[eltype(w)], 0)
+        push!(jobs, SthJob(_array(parent(src), size(src), _halo(src)),
+                           _array(dest, size(src), 0), sd, op,
+                           Base.unsafe_convert(Ptr{Cvoid}, pv),
+                           _bc(boundary(src)), _pad(padding(src))))
+    end
+    st = GC.@preserve keep jobs ccall((:sth_map_batch, libsth), Cint,
+                                      (Ptr{Cvoid}, Int32, Ptr{SthJob}, UInt32, Ptr{Cvoid}),
+                                      _context(), Int32(length(jobs)), jobs, UInt32(0), C_NULL)
+    if st == STH_ENOTSUP
+        foreach(c -> Stencils.gatherstencil!(c...), calls)
+    elseif st != STH_OK
+        _throw(st)
+    end
+    return calls
+end
+
+# ------------------------------------------------------------ serving loop
+"""
+    Server(hbm_budget; round_steps=8, serial=false)
+
+Continuous batching of iterated solves (`sth_server_*`): `submit!` queues
+`steps` calls of `A = mapstencil!(f, A)` on a SwitchingStencilArray over
+device arrays, `start!` starts the scheduler, `completions!` returns the
+finished requests (tag, times in server-clock seconds) and `drain!` waits for
+all of them.  Each request's final state, halo included, is copied into its
+result array.
+"""
+mutable struct Server
+    h::Ptr{Cvoid}
+    keep::Dict{UInt64,Any}
+    next::UInt64
+end
+
+function Server(hbm_budget::Integer; round_steps::Integer=8, serial::Bool=false)
+    cfg = Ref(SthServerConfig(Csize_t(hbm_budget), 0, Int32(round_steps), serial ? UInt32(1) : UInt32(0), 0))
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    st = ccall((:sth_server_create, libsth), Cint,
+               (Ptr{Cvoid}, Ref{SthServerConfig}, Ptr{Cvoid}, Ref{Ptr{Cvoid}}), _context(), cfg, C_NULL, h)
+    st == STH_OK || _throw(st)
+    srv = Server(h[], Dict{UInt64,Any}(), UInt64(1))
+    finalizer(close!, srv)
+    return srv
+end
+
+"""
+    submit!(srv, f, A::SwitchingStencilArray, steps; arrival=0.0, result=similar(source(A)))
+
+Queue a request; returns its tag.  `result` receives the final state (the
+parent array, halo included).
+"""
+function submit!(srv::Server, f::AccelOp, A::SwitchingStencilArray{R,T,N,<:ROCArray}, steps::Integer;
+                 arrival::Real=0.0, result::ROCArray=similar(source(A))) where {R,T,N}
+    op = _op(f, T)
+    pv = _padval(boundary(A), T)
+    (op === nothing || pv === nothing) && throw(ArgumentError("$f is not served on $T"))
+    offs, kind, r, n, l = _stencil_desc(stencil(A))
+    w = _weights(stencil(A), T)
+    h = padding(A) isa Halo ? R : 0
+    lsz = size(source(A)) .- 2h
+    tag = srv.next
+    srv.next += 1
+    st = GC.@preserve offs w pv begin
+        sd = SthStencil(kind, r, n, l, pointer(offs),
+                        w === nothing ? C_NULL : Ptr{Cvoid}(pointer(w)),
+                        w === nothing ? Int32(-1) : DTYPES[eltype(w)], 0)
+        q = Ref(SthRequest(_array(source(A), lsz, h), sd, op, Base.unsafe_convert(Ptr{Cvoid}, pv),
+                           _bc(boundary(A)), _pad(padding(A)), Int64(steps),
+                           Ptr{Cvoid}(UInt(pointer(result))), Float64(arrival), tag))
+        ccall((:sth_server_submit, libsth), Cint, (Ptr{Cvoid}, Ref{SthRequest}), srv.h, q)
+    end
+    st == STH_OK || _throw(st)
+    srv.keep[tag] = (A, result)   # the state is copied at admission, the result at the end
+    return tag
+end
+
+start!(srv::Server) = (st = ccall((:sth_server_start, libsth), Cint, (Ptr{Cvoid},), srv.h);
+                       st == STH_OK || _throw(st); srv)
+
+"""
+    completions!(srv; wait_ms=0) -> Vector{SthCompletion}
+
+Completions not returned before; waits up to `wait_ms` for the first.
+"""
+function completions!(srv::Server; wait_ms::Integer=0)
+    out = Vector{SthCompletion}(undef, 256)
+    n = Ref{Int32}(0)
+    st = ccall((:sth_server_poll, libsth), Cint,
+               (Ptr{Cvoid}, Ptr{SthCompletion}, Int32, Ref{Int32}, Int32),
+               srv.h, out, Int32(length(out)), n, Int32(wait_ms))
+    st == STH_OK || _throw(st)
+    done = out[1:n[]]
+    foreach(c -> delete!(srv.keep, c.tag), done)
+    return done
+end
+
+function drain!(srv::Server; timeout::Real=600.0)
+    st = ccall((:sth_server_drain, libsth), Cint, (Ptr{Cvoid}, Float64), srv.h, Float64(timeout))
+    st == STH_OK || _throw(st)
+    return srv
+end
+
+function close!(srv::Server)
+    if srv.h != C_NULL
+        ccall((:sth_server_destroy, libsth), Cint, (Ptr{Cvoid},), srv.h)
+        srv.h = C_NULL
+    end
+    return nothing
 end
 
 end # module
