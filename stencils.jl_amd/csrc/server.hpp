@@ -159,16 +159,37 @@ struct sth_server {
       st.offsets = q.offs.data();
       st.weights = q.weights.empty() ? nullptr : q.weights.data();
       const sth_array a = buffer_desc(q, 0), b = buffer_desc(q, 1);
-      for (int c = 0; c < 2 && rc == STH_OK; ++c) {
+      int prc = STH_OK;
+      for (int c = 0; c < 2 && prc == STH_OK; ++c) {
         std::vector<sth::Variant> cands;
         std::string group;
-        rc = sth::make_plan(c ? &b : &a, c ? &a : &b, &st, q.r.boundary, &q.padval, q.r.padding, &q.r.op, 0,
-                            &q.plan[c], &cands, &group);
+        prc = sth::make_plan(c ? &b : &a, c ? &a : &b, &st, q.r.boundary, &q.padval, q.r.padding, &q.r.op, 0,
+                             &q.plan[c], &cands, &group);
         int idx = 0;
-        if (rc == STH_OK) rc = tune_pick(ctx, q.plan[c], cands, group, 0, stream, false, true, &idx);
-        if (rc == STH_OK) sth::apply_variant(q.plan[c], cands[idx]);
+        if (prc == STH_OK) prc = tune_pick(ctx, q.plan[c], cands, group, 0, stream, false, true, &idx);
+        if (prc == STH_OK) sth::apply_variant(q.plan[c], cands[idx]);
       }
-      if (rc) return rc;
+      if (prc == STH_EINVAL || prc == STH_ENOTSUP) {
+        // this request cannot run (submit validated it against the same
+        // layout, so this is a safeguard): it completes with that status and
+        // the others go on
+        release(o0, q.bytes_one);
+        release(o1, q.bytes_one);
+        sth_completion c{};
+        c.tag = q.r.tag;
+        c.status = prc;
+        c.arrival = q.r.arrival;
+        c.admitted = t;
+        c.finished = t;
+        c.steps = q.r.steps;
+        c.cells = q.cells;
+        completed.push_back(c);
+        stats.completed += 1;
+        pending.pop_front();
+        done_cv.notify_all();
+        continue;
+      }
+      if (prc) return prc;
       // both buffers start as the state, as a SwitchingStencilArray's source
       // and dest do (src/array.jl:593-598)
       for (const size_t o : {o0, o1}) {

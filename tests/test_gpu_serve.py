@@ -149,6 +149,41 @@ def test_large_requests_two_step_launches(gpu):
         assert torch.equal(c.result, ref), (S, steps)  # halo included
 
 
+def test_invalid_request_rejected_at_submit(gpu):
+    """A request the library cannot run (reach >= axis, the reference's
+    ArgumentError) is rejected when submitted, through the C ABI past the
+    Python-side checks; the server keeps serving the others."""
+    import ctypes
+    import torch
+    from stencils_hip import _abi, _lib
+    from stencils_hip.gather import _Call, stencil_descriptor
+    dev = torch.device("cuda")
+    srv = sh.Server(hbm_budget=1 << 26)
+    good = sh.colmajor_empty((20, 20, 20), torch.float32, dev)
+    good.copy_(torch.rand((20, 20, 20), device=dev))
+    S = sh.SwitchingStencilArray(good, sh.VonNeumann(1, 3), boundary=sh.Wrap())
+    tag_ok = srv.submit(sh.Laplacian(0.15), S, 5)
+    bad = sh.colmajor_empty((6, 6, 1), torch.float32, dev)
+    bad.zero_()
+    out = sh.colmajor_empty((6, 6, 1), torch.float32, dev)
+    keep = _Call()
+    q = _abi.SthRequest()
+    q.state = _abi.make_array(bad.data_ptr(), _abi.F32, (6, 6, 1), (1, 6, 36), 0)
+    q.stencil = stencil_descriptor(sh.VonNeumann(1, 3), _abi.F32, keep)
+    q.op = sh.Laplacian(0.15).op(0)
+    q.boundary, q.padding, q.steps = sh.Wrap().code, _abi.CONDITIONAL, 3
+    q.result = out.data_ptr()
+    q.tag = 999
+    with pytest.raises(ValueError, match="radius"):
+        _lib.check(_lib.lib().sth_server_submit(srv._h, ctypes.byref(q)))
+    srv.start()
+    srv.drain(timeout=60)
+    done = {c.tag: c for c in srv.completions()}
+    srv.close()
+    assert set(done) == {tag_ok} and done[tag_ok].status == _abi.STH_OK
+    assert torch.equal(done[tag_ok].result, _alone(sh.Laplacian(0.15), S, 5))
+
+
 def test_request_errors(gpu):
     import torch
     dev = torch.device("cuda")
