@@ -560,6 +560,8 @@ struct BatchSlot {
 // used slot is overwritten, once its last launch has finished with it.
 constexpr int kBatchSlots = 32;
 
+constexpr size_t kMaxGraphs = 512;  // cached graph executables per context
+
 struct sth_ctx {
   int device = 0;
   int cus = 0;
@@ -1067,6 +1069,13 @@ int sth_iterate(sth_ctx* ctx, const sth_array* a, const sth_array* b, const sth_
       e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
       if (e != hipSuccess) return fail(STH_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+      if (ctx->graphs.size() >= kMaxGraphs) {
+        // bound the cache (a long-running caller iterating ever new buffers
+        // would grow it without end): wait for the device, then drop it all
+        STH_HIP(hipDeviceSynchronize());
+        for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+        ctx->graphs.clear();
+      }
       ctx->graphs.emplace(key, x);
       *exec = x;
       return STH_OK;

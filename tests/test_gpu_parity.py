@@ -254,6 +254,26 @@ def test_single_steps_match_iterate(gpu):
     np.testing.assert_array_equal(np.asarray(S1), np.asarray(S2))
 
 
+def test_graph_cache_bound(gpu, oracle):
+    """Iterations on many distinct buffers fill the per-context graph cache
+    past its bound (it is then cleared); every result stays exact."""
+    import torch
+    dev = torch.device("cuda")
+    rng = np.random.default_rng(31)
+    alive = []  # distinct device buffers, so every call plans new graphs
+    for i in range(300):
+        A = rng.random((16 + i % 5, 12))
+        t = sh.colmajor_empty(A.shape, torch.float64, dev)
+        t.copy_(torch.from_numpy(A))
+        G = sh.SwitchingStencilArray(t, sh.Window(1), boundary=sh.Reflect())
+        alive.append(G)
+        O = sh.SwitchingStencilArray(A.copy(order="F"), sh.Window(1), boundary=sh.Reflect())
+        # 8 steps: two graphs per call (6 replayed steps and 1), 600 in all
+        G = sh.mapstencil_steps(sh.mean, G, 8, graph=True)
+        O = oracle.mapstencil_steps(sh.mean, O, 8)
+        np.testing.assert_array_equal(G.source.cpu().numpy(), O.source, err_msg=f"call {i}")
+
+
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
 def test_heat_3d_iterate(gpu, oracle, dt):
     A = _np_input((40, 33, 29), dt, 13)
