@@ -8,6 +8,7 @@ the timed steps run as replays of a captured hipGraph.  Prints one JSON line.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 """
 import argparse
+import ctypes
 import json
 import os
 import re
@@ -230,7 +231,9 @@ def batched_workload(sh, torch, n: int = 512, seed: int = 4):
 
 
 def bench_batched(sh, torch):
-    """Config 4: one persistent launch (sth_map_batch) vs one launch per array."""
+    """Config 4: sth_map_batch (one launch per type, shape and op) against one
+    launch per array: asynchronous from C, asynchronous from Python, and one
+    blocking call per array."""
     calls = batched_workload(sh, torch)
     plan = sh.BatchPlan(calls)
     cells = plan.cells
@@ -254,6 +257,20 @@ def bench_batched(sh, torch):
         for f, d, sa in calls:
             sh.mapstencil_(f, d, sa, flags=sh._abi.ASYNC)
     wall_p, gpu_p = timed(python_loop, 2)
+
+    # One blocking sth_map call per array, descriptors prebuilt: the
+    # reference's per-array mapstencil loop, where every call synchronises
+    # (src/gatherstencil.jl:100), without Python-side descriptor building.
+    L = sh._lib.lib()
+    ctx = sh._lib.context(torch.cuda.current_device())
+    strm = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def blocking_loop():
+        for i in range(plan.n):
+            j = plan.jobs[i]
+            sh._lib.check(L.sth_map(ctx, ctypes.byref(j.src), ctypes.byref(j.dst), ctypes.byref(j.stencil),
+                                    j.boundary, j.padval, j.padding, ctypes.byref(j.op), 0, strm))
+    wall_k, _ = timed(blocking_loop, 2)
     # spot check: the batched results equal single calls
     plan.run()
     for f, d, sa in calls[:8]:
@@ -266,7 +283,9 @@ def bench_batched(sh, torch):
             "per_array_launches_c": {"ms": wall_s * 1e3, "gpu_ms": gpu_s * 1e3,
                                      "cells_per_s": cells / wall_s},
             "per_array_python_loop": {"ms": wall_p * 1e3, "cells_per_s": cells / wall_p},
-            "speedup_vs_c_loop": wall_s / wall_b, "speedup_vs_python_loop": wall_p / wall_b}
+            "per_array_blocking_calls": {"ms": wall_k * 1e3, "cells_per_s": cells / wall_k},
+            "speedup_vs_c_loop": wall_s / wall_b, "speedup_vs_python_loop": wall_p / wall_b,
+            "speedup_vs_blocking_calls": wall_k / wall_b}
 
 
 def served_requests(sh, torch, n: int, tdt, seed: int = 5, rate: float = 200.0):
