@@ -60,7 +60,7 @@ struct Fused2Entry {
   int ndim;
   const int8_t* offs;
   LaunchFn fn;
-  int rows;  // rows per wave of the z-marching kernel
+  int rows;  // 3-D: rows per wave of the z-marching kernel; 2-D: cells per lane
 };
 std::vector<Fused2Entry>& fused2_registry();
 

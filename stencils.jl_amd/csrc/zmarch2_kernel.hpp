@@ -69,9 +69,11 @@ __device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, u
     return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0));
   } else if constexpr (sizeof(T) == 4) {
     return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
-  } else {
-    static_assert(sizeof(T) == 8, "2-, 4- or 8-byte elements");
+  } else if constexpr (sizeof(T) == 8) {
     return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  } else {
+    static_assert(sizeof(T) == 16, "2-, 4-, 8- or 16-byte accesses");
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
   }
 }
 template <typename T>
@@ -80,10 +82,13 @@ __device__ __forceinline__ void buf_store(T v, __amdgpu_buffer_rsrc_t r, uint32_
     __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, voff, soff, 0);
   } else if constexpr (sizeof(T) == 4) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, voff, soff, 0);
-  } else {
-    static_assert(sizeof(T) == 8, "2-, 4- or 8-byte elements");
+  } else if constexpr (sizeof(T) == 8) {
     using V2 = unsigned int __attribute__((ext_vector_type(2)));
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, voff, soff, 0);
+  } else {
+    static_assert(sizeof(T) == 16, "2-, 4-, 8- or 16-byte accesses");
+    using V4 = unsigned int __attribute__((ext_vector_type(4)));
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V4, v), r, voff, soff, 0);
   }
 }
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uint32_t bytes) {

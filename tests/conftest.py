@@ -18,9 +18,10 @@ sys.path.insert(0, str(REPO / "stencils.jl_amd"))
 sys.path.insert(0, str(REPO / "oracle"))
 sys.path.insert(0, str(REPO / "tests"))
 
-# Two-step 3-D launches run by default only on large arrays; tests run them
+# Two-step launches run by default only on large arrays; tests run them
 # on every eligible array so small shapes cover their boundary cases.
 os.environ.setdefault("STH_FUSE3D_MIN_CELLS", "0")
+os.environ.setdefault("STH_FUSE2D_MIN_CELLS", "0")
 
 GOLDEN = json.loads((REPO / "tests" / "golden" / "reference_goldens.json").read_text())
 
