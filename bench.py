@@ -413,6 +413,42 @@ def bench_heat_iterate(sh, torch, n: int = 512, steps: int = 200):
     return res
 
 
+def bench_heat_iterate_2d(sh, torch, steps: int = 100):
+    """2-D heat iteration (VonNeumann(1) Laplacian(0.2), Wrap) as `steps`
+    hipGraph-replayed steps: two steps per launch (ymarch2_kernel) against one
+    streaming step per launch (STH_NO_FUSE); f32 and bf16 on 16384^2, f64 on
+    8192^2 (a 16384^2 f64 grid is past the two-step kernel's 2 GiB offsets)."""
+    f = sh.Laplacian(0.2)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    stream = torch.cuda.current_stream()
+    res = {"workload": f"heat 2-D VonNeumann(1) Laplacian(0.2) Wrap, {steps} steps, hipGraph"}
+    for name, tdt, n in (("f32", torch.float32, 16384), ("bf16", torch.bfloat16, 16384),
+                         ("f64", torch.float64, 8192)):
+        A = sh.colmajor_empty((n, n), tdt, dev)
+        A.copy_(torch.rand((n, n), device=dev).to(tdt))
+        S = sh.SwitchingStencilArray(A, sh.VonNeumann(1), boundary=sh.Wrap())
+        t = {}
+        for key, fl in (("two_step", 0), ("one_step", sh._abi.NO_FUSE)):
+            S = sh.mapstencil_steps(f, S, 8, flags=sh._abi.ASYNC | fl)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            e0.record(stream)
+            S = sh.mapstencil_steps(f, S, steps, flags=sh._abi.ASYNC | fl)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            t[key] = e0.elapsed_time(e1) / steps * 1e3
+        es = A.element_size()
+        res[name] = {"n": n, "two_step_us_per_step": t["two_step"], "one_step_us_per_step": t["one_step"],
+                     "speedup": t["one_step"] / t["two_step"],
+                     "two_step_achieved_gbs": 2 * 2 * es * n * n / (2 * t["two_step"] * 1e-6) / 1e9,
+                     "one_step_achieved_gbs": 2 * es * n * n / (t["one_step"] * 1e-6) / 1e9}
+        if name == "f32":
+            res[name]["two_step_traffic"] = profiled_traffic("ymarch2_kernel_heat2d_16384.md")
+        del A, S
+        torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -423,7 +459,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the secondary workloads (mean blur, Gaussian)")
-    ap.add_argument("--only", choices=["heat3d", "served", "batched", "gaussian", "meanblur"],
+    ap.add_argument("--only", choices=["heat3d", "heat2d", "served", "batched", "gaussian", "meanblur"],
                     help="run one secondary workload alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -438,6 +474,7 @@ def main():
     torch.cuda.set_device(int(os.environ.get("LOCAL_RANK", 0)))
     if args.only:
         run_one = {"heat3d": lambda: bench_heat_iterate(sh, torch, steps=args.steps),
+                   "heat2d": lambda: bench_heat_iterate_2d(sh, torch),
                    "served": lambda: bench_served(sh, torch),
                    "batched": lambda: bench_batched(sh, torch),
                    "gaussian": lambda: bench_gaussian(sh, torch),
@@ -523,6 +560,7 @@ def main():
                           "gaussian_7x7": bench_gaussian(sh, torch),
                           "batched_uneven": bench_batched(sh, torch),
                           "heat_3d_iterate": bench_heat_iterate(sh, torch),
+                          "heat_2d_iterate": bench_heat_iterate_2d(sh, torch),
                           "served_heat_3d": bench_served(sh, torch)}
     print(json.dumps(out), flush=True)
 

@@ -188,8 +188,10 @@ int sth_map(sth_ctx* ctx, const sth_array* src, const sth_array* dst,
  * buffers a (source) and b (dest) on device.  On return *current is 0 if
  * `a` holds the newest state, 1 if `b` does (the switched wrapper's source).
  * With STH_GRAPH the steps are captured once into hipGraphs and replayed.
- * Life rules (2-D) and 3-D radius-1 catalogue stencils under Wrap or Remove
- * (arrays of 128^3 cells or more) advance two steps per launch in the steady
+ * Life rules (2-D), 3-D radius-1 catalogue stencils under Wrap or Remove
+ * (arrays of 128^3 cells or more) and 2-D radius-1 catalogue stencils in float
+ * types under Wrap or Remove (2^21 cells or more, rows on 16-byte boundaries)
+ * advance two steps per launch in the steady
  * steps (temporal blocking; STH_NO_FUSE turns it off) and finish with single
  * steps, so both buffers end exactly as the step-by-step reference leaves them.
  */
