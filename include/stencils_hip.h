@@ -264,9 +264,9 @@ int sth_map_batch(sth_ctx* ctx, int32_t njobs, const sth_job* jobs, uint32_t fla
  * run), and advances every admitted request together: each step of a round
  * is one batched launch over all of them (sth_map_batch's kernels), so
  * requests that arrive while others run join at the next round.  Requests
- * of at least 128^3 cells with a two-step kernel (3-D radius-1 stencils,
- * Wrap or Remove) run their own two-step launches on their own streams
- * instead.  Finished states (halo included) are copied to the request's
+ * of at least 2^21 cells with a two-step kernel (radius-1 catalogue stencils
+ * in 2-D and 3-D under Wrap or Remove, Life) run their own two-step launches
+ * on their own streams instead.  Finished states (halo included) are copied to the request's
  * `result` and their buffers freed.
  * STH_SERVER_SERIAL instead serves one request at a time with one plain
  * launch per step (the unbatched path, for comparison).

@@ -14,7 +14,7 @@
 //   3. advance -- K = min(round_steps, fewest remaining steps) steps; each
 //                 step is one batched launch over the active requests
 //                 (requests join and leave only between rounds), buffers
-//                 ping-pong per request.  Requests of 128^3 cells or more
+//                 ping-pong per request.  Requests of 2^21 cells or more
 //                 that have a two-step kernel run their own launches
 //                 instead, two steps each, up to 4 x round_steps steps per
 //                 round.  Requests that reach their step count get
@@ -263,7 +263,8 @@ struct sth_server {
   }
 
   // --- 3. one round (called without `mu`: only the scheduler touches `active`)
-  // A request with a two-step kernel and at least 2^21 cells (128^3) runs as
+  // A request with a two-step kernel (3-D and 2-D radius-1 stencils, Life)
+  // and at least 2^21 cells runs as
   // launches of its own: two steps per launch, and more steps per round than
   // the shared batch.  Below that size batching the steps pays more.
   // (STH_SERVER_SOLO_CELLS overrides the size)
@@ -285,6 +286,15 @@ struct sth_server {
   // halo included, is the step-by-step one.
   int advance_solo(Req& q, int64_t n, int64_t* launches) {
     const hipStream_t st = solo_streams[q.lane];
+    if (q.done == 0 && n > 0) {
+      // the first step is a single step, as in sth_iterate: the two-step
+      // kernels assume data this library wrote (Life: bytes already 0 or 1)
+      if (int rc = sth::run_plan(q.plan[q.cur], st)) return rc;
+      q.cur ^= 1;
+      q.done += 1;
+      *launches += 1;
+      n -= 1;
+    }
     const int64_t rem = q.r.steps - q.done;
     const int64_t fused = n == rem ? std::max<int64_t>(0, (n - 2) / 2) : n / 2;
     for (int64_t i = 0; i < fused; ++i) {

@@ -5,7 +5,7 @@ A request is `steps` calls of ``A = mapstencil!(f, A)`` on a
 SwitchingStencilArray (src/gatherstencil.jl:77-83).  The server copies the
 initial state into its arena when the request is admitted, advances every
 admitted request together (one batched launch per step for all of them;
-requests of 128^3 cells or more with a two-step kernel run their own
+requests of 2^21 cells or more with a two-step kernel run their own
 launches) and copies the final state to the request's result buffer.
 
     srv = Server(hbm_budget=1 << 30)

@@ -140,6 +140,17 @@ def test_large_requests_two_step_launches(gpu):
         A.copy_(torch.rand(shape, device=dev))
         reqs.append((sh.Laplacian(0.15), sh.SwitchingStencilArray(A, sh.VonNeumann(1, 3), boundary=bc,
                                                                   padding=pad), steps, 0.0))
+    # 2-D requests of 2^21 cells with two-step kernels: heat (f32) and Life
+    # with bytes other than 0 / 1 (the first served step must normalise them)
+    H = sh.colmajor_empty((2048, 1024), torch.float32, dev)
+    H.copy_(torch.rand((2048, 1024), device=dev))
+    reqs.append((sh.Laplacian(0.2), sh.SwitchingStencilArray(H, sh.VonNeumann(1), boundary=sh.Wrap()), 23, 0.0))
+    rng = np.random.default_rng(4)
+    G = (rng.random((2048, 1024)) < 0.3).astype(np.uint8) * rng.integers(1, 6, (2048, 1024)).astype(np.uint8)
+    Gt = sh.colmajor_empty((2048, 1024), torch.uint8, dev)
+    Gt.copy_(torch.from_numpy(G))
+    reqs.append((sh.LifeRule(), sh.SwitchingStencilArray(Gt, sh.Moore(1), boundary=sh.Wrap(), padding=sh.Halo()),
+                 21, 0.0))
     reqs += _mixed_requests(3, 6)
     done, tags, stats = _serve(reqs, hbm_budget=1 << 30, round_steps=4)
     assert len(done) == len(reqs)
