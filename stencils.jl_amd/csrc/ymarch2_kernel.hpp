@@ -94,8 +94,11 @@ __global__ __launch_bounds__(256) void ymarch2_kernel(MapParams p) {
   const __amdgpu_buffer_rsrc_t dst = buf_rsrc(p.dst.base, static_cast<uint32_t>(dext));
   constexpr uint32_t kDrop = 0x80000000u;
 
+  // mapped coordinate of c; cells beyond the two-cell rim feed no stored
+  // result and are clamped into it first (n >= 3, so every mapped rim cell
+  // lies inside the array)
   auto map_c = [&](int32_t c, int32_t n) -> int32_t {
-    c = c < -V ? -V : (c > n + V - 1 ? n + V - 1 : c);
+    c = c < -2 ? -2 : (c > n + 1 ? n + 1 : c);
     if (c < 0) return REMOVE ? 0 : c + n;
     if (c >= n) return REMOVE ? 0 : c - n;
     return c;
