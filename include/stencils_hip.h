@@ -188,9 +188,10 @@ int sth_map(sth_ctx* ctx, const sth_array* src, const sth_array* dst,
  * buffers a (source) and b (dest) on device.  On return *current is 0 if
  * `a` holds the newest state, 1 if `b` does (the switched wrapper's source).
  * With STH_GRAPH the steps are captured once into hipGraphs and replayed.
- * Life rules (2-D), 3-D radius-1 catalogue stencils under Wrap or Remove
- * (arrays of 128^3 cells or more) and 2-D radius-1 catalogue stencils in float
- * types under Wrap or Remove (2^21 cells or more, rows on 16-byte boundaries)
+ * Life rules (2-D), 3-D radius-1 catalogue stencils under Wrap, Remove or
+ * Reflect (arrays of 128^3 cells or more) and 2-D radius-1 catalogue stencils
+ * in float types under Wrap, Remove or Reflect (2^21 cells or more, rows on
+ * 16-byte boundaries)
  * advance two steps per launch in the steady
  * steps (temporal blocking; STH_NO_FUSE turns it off) and finish with single
  * steps, so both buffers end exactly as the step-by-step reference leaves them.
@@ -265,7 +266,7 @@ int sth_map_batch(sth_ctx* ctx, int32_t njobs, const sth_job* jobs, uint32_t fla
  * is one batched launch over all of them (sth_map_batch's kernels), so
  * requests that arrive while others run join at the next round.  Requests
  * of at least 2^21 cells with a two-step kernel (radius-1 catalogue stencils
- * in 2-D and 3-D under Wrap or Remove, Life) run their own two-step launches
+ * in 2-D and 3-D, Life) run their own two-step launches
  * on their own streams instead.  Finished states (halo included) are copied to the request's
  * `result` and their buffers freed.
  * STH_SERVER_SERIAL instead serves one request at a time with one plain

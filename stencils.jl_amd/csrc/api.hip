@@ -442,7 +442,8 @@ int make_plan(const sth_array* src, const sth_array* dst, const sth_stencil* st,
   apply_variant(pl, cands[0]);
   if (cands_out) *cands_out = std::move(cands);
 
-  // two steps per launch for 3-D radius-1 catalogue stencils (Wrap / Remove)
+  // two steps per launch for 3-D radius-1 catalogue stencils (Wrap / Remove /
+  // Reflect)
   // on arrays large enough for the z-marching kernel to beat two streaming
   // steps: from 128^3 cells it measured 1.0-1.5x faster per step, from 256^3
   // 1.5-2.8x (STH_FUSE3D_MIN_CELLS overrides the threshold)
@@ -451,7 +452,7 @@ int make_plan(const sth_array* src, const sth_array* dst, const sth_stencil* st,
     return e ? std::atoll(e) : (int64_t(1) << 21);
   }();
   if (!pl.fn_fused2 && !(flags & (STH_NO_FUSE | STH_GENERIC)) && src->ndim == 3 && st->ndim == 3 &&
-      (bc == STH_WRAP || bc == STH_REMOVE) && n[0] >= 3 && n[1] >= 3 && n[2] >= 3 &&
+      (bc == STH_WRAP || bc == STH_REMOVE || bc == STH_REFLECT) && n[0] >= 3 && n[1] >= 3 && n[2] >= 3 &&
       n[0] * n[1] * n[2] >= fuse3d_min &&
       zm2_extent(pl.p.src, dtype_size(sdt)) < kZm2MaxBytes && zm2_extent(pl.p.dst, dtype_size(odt)) < kZm2MaxBytes) {
     for (const auto& e : fused2_registry()) {
@@ -467,7 +468,8 @@ int make_plan(const sth_array* src, const sth_array* dst, const sth_stencil* st,
     }
   }
 
-  // two steps per launch for 2-D radius-1 catalogue stencils (Wrap / Remove)
+  // two steps per launch for 2-D radius-1 catalogue stencils (Wrap / Remove /
+  // Reflect)
   // on arrays whose rows start on 16-byte boundaries (the y-marching kernel
   // loads whole vectors) and of 2^21 cells or more: 1.3-1.7x faster per step
   // from 4096^2, mixed (0.6-1.3x) at 1024^2 (STH_FUSE2D_MIN_CELLS overrides)
@@ -479,7 +481,8 @@ int make_plan(const sth_array* src, const sth_array* dst, const sth_stencil* st,
     return reinterpret_cast<uintptr_t>(a.base) % 16 == 0 && (a.s1 * es) % 16 == 0;
   };
   if (!pl.fn_fused2 && !(flags & (STH_NO_FUSE | STH_GENERIC)) && src->ndim == 2 && st->ndim == 2 &&
-      (bc == STH_WRAP || bc == STH_REMOVE) && n[0] >= 3 && n[1] >= 3 && n[0] * n[1] >= fuse2d_min &&
+      (bc == STH_WRAP || bc == STH_REMOVE || bc == STH_REFLECT) && n[0] >= 3 && n[1] >= 3 &&
+      n[0] * n[1] >= fuse2d_min &&
       sdt == odt && rows16(pl.p.src, dtype_size(sdt)) && rows16(pl.p.dst, dtype_size(odt)) &&
       zm2_extent(pl.p.src, dtype_size(sdt)) < kZm2MaxBytes && zm2_extent(pl.p.dst, dtype_size(odt)) < kZm2MaxBytes) {
     for (const auto& e : fused2_registry()) {

@@ -20,7 +20,9 @@
 // fold expressions in offset order, generation t+1 is rounded to the storage
 // type as a store would round it, Wrap outside cells are translations of
 // inside ones (src/array.jl:167-179) and Remove outside cells read padval in
-// every generation (src/array.jl:133-135); Reflect is not fused.
+// every generation (src/array.jl:133-135).  Under Reflect the outside
+// generation-t+1 cells next to the array are replaced by the cells they
+// mirror (src/array.jl:154-166) before generation t+2 reads them.
 #pragma once
 
 #include "tile_kernel.hpp"
@@ -58,8 +60,19 @@ __device__ __forceinline__ void ym2_fold_terms(typename AccOf<S, OPK>::type& acc
   }
 }
 
-template <typename S, int OPK, class Sh, bool REMOVE>
+// The v-th element of a lane's vector for a wave-uniform index v.
+template <int V, typename Val>
+__device__ __forceinline__ Val pick(const Val (&q)[V], int v) {
+  Val r = q[0];
+#pragma unroll
+  for (int k = 1; k < V; ++k)
+    if (k == v) r = q[k];
+  return r;
+}
+
+template <typename S, int OPK, class Sh, int BC>
 __global__ __launch_bounds__(256) void ymarch2_kernel(MapParams p) {
+  constexpr bool REMOVE = BC == STH_REMOVE, REFLECT = BC == STH_REFLECT;
   using Raw = typename Elem<S>::raw;
   using Val = typename Staged<S>::type;
   using A = typename AccOf<S, OPK>::type;
@@ -99,8 +112,8 @@ __global__ __launch_bounds__(256) void ymarch2_kernel(MapParams p) {
   // lies inside the array)
   auto map_c = [&](int32_t c, int32_t n) -> int32_t {
     c = c < -2 ? -2 : (c > n + 1 ? n + 1 : c);
-    if (c < 0) return REMOVE ? 0 : c + n;
-    if (c >= n) return REMOVE ? 0 : c - n;
+    if (c < 0) return REMOVE ? 0 : (REFLECT ? -c : c + n);
+    if (c >= n) return REMOVE ? 0 : (REFLECT ? 2 * n - 2 - c : c - n);
     return c;
   };
   // this lane's columns: x0 .. x0 + V - 1
@@ -150,6 +163,29 @@ __global__ __launch_bounds__(256) void ymarch2_kernel(MapParams p) {
     e[0] = lane_shift<-1>(q[V - 1]);
     e[1] = lane_shift<1>(q[0]);
   };
+  // Reflect: the generation-(t+1) cells just outside the array, x = -1 and
+  // x = n0, take the values of the cells they mirror, x = 1 and x = n0 - 2 --
+  // the reference reads those cells there (src/array.jl:154-166), and a fold
+  // over the mirrored neighbourhood would sum in the reverse order.
+  auto reflect_columns = [&](Val (&h)[V]) {
+    if (X0 == 0) {  // wave-uniform
+      const Val from_right = lane_shift<1>(h[1]);  // lane l: lane l+1's element 1
+      if (lane == 0) h[V - 1] = from_right;
+    }
+    const int32_t pr = n0 - (X0 - V);  // x = n0 in this wave's cells (uniform)
+    if (pr >= 0 && pr < 64 * V) {
+      const int lr = pr / V, ir = pr % V;
+      const int pm = pr - 2, lm = pm / V, im = pm % V;  // x = n0 - 2 (pm >= 0)
+      const Val same = pick<V>(h, im);
+      const Val prev = lane_shift<-1>(same);  // lane l: lane l-1's element im
+      const Val v = lm == lr ? same : prev;
+      if (lane == lr) {
+#pragma unroll
+        for (int k = 0; k < V; ++k)
+          if (k == ir) h[k] = v;
+      }
+    }
+  };
 
   const int nit = (ye - yb) + 2;  // generation t+1 rows yb-1 .. ye; generation-t rows 0 .. nit+1
 #pragma unroll
@@ -187,23 +223,40 @@ __global__ __launch_bounds__(256) void ymarch2_kernel(MapParams p) {
           for (int i = 0; i < V; ++i)
             if (xout[i] || yout) h[i] = pv;
         }
+        if constexpr (REFLECT) reflect_columns(h);
         edges(h, e1[u & 3]);
       }
       // generation t+2 on row yb - 2 + j
       if (j >= 2) {
         const int64_t y = yb - 2 + j;
-        const auto& q0 = g1[(u + 2) & 3];
         const auto& q1 = g1[(u + 3) & 3];
-        const auto& q2 = g1[u & 3];
+        RawVec<ORaw, V> out;
+        auto fold_row = [&](const Val (&q0)[V], const Val (&q2)[V], const Val (&lm)[3], const Val (&rp)[3]) {
+#pragma unroll
+          for (int i = 0; i < V; ++i) {
+            A acc = A(0);
+            uint32_t cnt = 0;
+            ym2_fold_terms<S, OPK, Sh, V, 0>(acc, cnt, q0, q1, q2, lm, rp, i, p);
+            out.e[i] = narrow<O>(fold_finish<S, OPK>(acc, cnt, q1[i], Sh::L, p));
+          }
+        };
         const Val lm[3] = {e1[(u + 2) & 3][0], e1[(u + 3) & 3][0], e1[u & 3][0]};
         const Val rp[3] = {e1[(u + 2) & 3][1], e1[(u + 3) & 3][1], e1[u & 3][1]};
-        RawVec<ORaw, V> out;
+        // Reflect: rows y = -1 and y = n1 read rows 1 and n1 - 2 (a
+        // wave-uniform branch taken on the array's first and last rows only)
+        if (REFLECT && (y == 0 || y == n1 - 1)) {
+          const bool top = y == 0, bottom = y == n1 - 1;
+          Val q0[V], q2[V];
 #pragma unroll
-        for (int i = 0; i < V; ++i) {
-          A acc = A(0);
-          uint32_t cnt = 0;
-          ym2_fold_terms<S, OPK, Sh, V, 0>(acc, cnt, q0, q1, q2, lm, rp, i, p);
-          out.e[i] = narrow<O>(fold_finish<S, OPK>(acc, cnt, q1[i], Sh::L, p));
+          for (int i = 0; i < V; ++i) {
+            q0[i] = top ? g1[u & 3][i] : g1[(u + 2) & 3][i];
+            q2[i] = bottom ? g1[(u + 2) & 3][i] : g1[u & 3][i];
+          }
+          const Val lr[3] = {top ? lm[2] : lm[0], lm[1], bottom ? lm[0] : lm[2]};
+          const Val rr[3] = {top ? rp[2] : rp[0], rp[1], bottom ? rp[0] : rp[2]};
+          fold_row(q0, q2, lr, rr);
+        } else {
+          fold_row(g1[(u + 2) & 3], g1[u & 3], lm, rp);
         }
         const uint32_t yoff = static_cast<uint32_t>(y * p.dst.s1 * es);
         if (store_vec) {
@@ -236,9 +289,11 @@ template <typename S, int OPK, class Sh>
 void launch_ymarch2(const MapParams& p, hipStream_t s) {
   const unsigned nb = static_cast<unsigned>(p.tiles_x) * p.tiles_y;
   if (p.bc == STH_REMOVE)
-    hipLaunchKernelGGL((ymarch2_kernel<S, OPK, Sh, true>), dim3(nb), dim3(256), 0, s, p);
+    hipLaunchKernelGGL((ymarch2_kernel<S, OPK, Sh, STH_REMOVE>), dim3(nb), dim3(256), 0, s, p);
+  else if (p.bc == STH_REFLECT)
+    hipLaunchKernelGGL((ymarch2_kernel<S, OPK, Sh, STH_REFLECT>), dim3(nb), dim3(256), 0, s, p);
   else
-    hipLaunchKernelGGL((ymarch2_kernel<S, OPK, Sh, false>), dim3(nb), dim3(256), 0, s, p);
+    hipLaunchKernelGGL((ymarch2_kernel<S, OPK, Sh, STH_WRAP>), dim3(nb), dim3(256), 0, s, p);
 }
 
 }  // namespace sth

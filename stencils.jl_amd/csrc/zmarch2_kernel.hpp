@@ -22,9 +22,11 @@
 //   - Wrap: an outside cell of generation t+1 is computed from its mapped
 //     neighbourhood -- Wrap is a translation, so that is the wrapped cell's
 //     value, with the same fold order;
-//   - Remove: outside cells read padval in every generation.
-// (Reflect mirrors the neighbourhood, which reverses the fold order of
-// floating-point sums, so it is not fused.)  Lanes, rows and planes beyond
+//   - Remove: outside cells read padval in every generation;
+//   - Reflect: a fold over the mirrored neighbourhood would sum in the
+//     reverse order, so the outside generation-t+1 cells next to the array
+//     are replaced by the cells they mirror, which the reference reads.
+// Lanes, rows and planes beyond
 // the two-cell rim of a partial tile are clamped into the rim; they feed no
 // stored result.
 #pragma once
@@ -117,8 +119,9 @@ __device__ __forceinline__ void zm2_fold_terms(typename AccOf<S, OPK>::type& acc
   }
 }
 
-template <typename S, int OPK, class Sh, int R, bool REMOVE>
+template <typename S, int OPK, class Sh, int R, int BC>
 __global__ __launch_bounds__(256) void zmarch2_kernel(MapParams p) {
+  constexpr bool REMOVE = BC == STH_REMOVE, REFLECT = BC == STH_REFLECT;
   using Raw = typename Elem<S>::raw;
   using Val = typename Staged<S>::type;
   using A = typename AccOf<S, OPK>::type;
@@ -159,8 +162,8 @@ __global__ __launch_bounds__(256) void zmarch2_kernel(MapParams p) {
   // mapped coordinate of c in [-2, n + 1] (beyond: clamped first)
   auto map_c = [&](int32_t c, int32_t n) -> int32_t {
     c = c < -2 ? -2 : (c > n + 1 ? n + 1 : c);
-    if (c < 0) return REMOVE ? 0 : c + n;
-    if (c >= n) return REMOVE ? 0 : c - n;
+    if (c < 0) return REMOVE ? 0 : (REFLECT ? -c : c + n);
+    if (c >= n) return REMOVE ? 0 : (REFLECT ? 2 * n - 2 - c : c - n);
     return c;
   };
   const int32_t x = X0 - 2 + lane;
@@ -225,13 +228,42 @@ __global__ __launch_bounds__(256) void zmarch2_kernel(MapParams p) {
           for (int r = 1; r < R - 1; ++r)
             if (xout || yout[r] || zout) h[r] = pv;
         }
+        if constexpr (REFLECT) {
+          // The generation-(t+1) cells just outside the array take the values
+          // of the cells they mirror (x = -1 <- 1, x = n0 <- n0 - 2, and the
+          // same in y), which is where the reference reads them
+          // (src/array.jl:154-166); planes are handled where generation t+2
+          // reads them.
+          if (X0 == 0) {  // x = -1 is lane 1, x = 1 lane 3
+#pragma unroll
+            for (int r = 1; r < R - 1; ++r) {
+              const Val v = lane_shift<1>(lane_shift<1>(h[r]));
+              if (lane == 1) h[r] = v;
+            }
+          }
+          const int ln = n0 - X0 + 2;  // lane of x = n0
+          if (ln >= 1 && ln <= 62) {
+#pragma unroll
+            for (int r = 1; r < R - 1; ++r) {
+              const Val v = lane_shift<-1>(lane_shift<-1>(h[r]));
+              if (lane == ln) h[r] = v;
+            }
+          }
+          if (Y0 == 0) h[1] = h[3];  // y = -1 <- y = 1
+          const int rn = n1 - Y0 + 2;  // row of y = n1
+#pragma unroll
+          for (int r = 3; r < R - 1; ++r)
+            if (r == rn) h[r] = h[r - 2];
+        }
       }
       // generation t+2 on plane zb - 2 + j, rows 2 .. R-3
       if (j >= 2) {
         const int64_t z = zb - 2 + j;
-        const auto& q0 = g1[(u + 2) & 3];
+        // Reflect: planes z = -1 and z = n2 read planes 1 and n2 - 2
+        const bool zlo = REFLECT && z == 0, zhi = REFLECT && z == n2 - 1;
+        const auto& q0 = zlo ? g1[u & 3] : g1[(u + 2) & 3];
         const auto& q1 = g1[(u + 3) & 3];
-        const auto& q2 = g1[u & 3];
+        const auto& q2 = zhi ? g1[(u + 2) & 3] : g1[u & 3];
         const uint32_t zoff = static_cast<uint32_t>(z * p.dst.s2 * eo);
 #pragma unroll
         for (int r = 2; r < R - 2; ++r) {
@@ -273,9 +305,11 @@ void launch_zmarch2(const MapParams& p, hipStream_t s) {
   constexpr int R = zmarch2_rows<typename Staged<S>::type>();
   const unsigned nb = static_cast<unsigned>(p.tiles_x) * p.tiles_y * p.tiles_z;
   if (p.bc == STH_REMOVE)
-    hipLaunchKernelGGL((zmarch2_kernel<S, OPK, Sh, R, true>), dim3(nb), dim3(256), 0, s, p);
+    hipLaunchKernelGGL((zmarch2_kernel<S, OPK, Sh, R, STH_REMOVE>), dim3(nb), dim3(256), 0, s, p);
+  else if (p.bc == STH_REFLECT)
+    hipLaunchKernelGGL((zmarch2_kernel<S, OPK, Sh, R, STH_REFLECT>), dim3(nb), dim3(256), 0, s, p);
   else
-    hipLaunchKernelGGL((zmarch2_kernel<S, OPK, Sh, R, false>), dim3(nb), dim3(256), 0, s, p);
+    hipLaunchKernelGGL((zmarch2_kernel<S, OPK, Sh, R, STH_WRAP>), dim3(nb), dim3(256), 0, s, p);
 }
 
 }  // namespace sth

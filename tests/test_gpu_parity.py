@@ -288,7 +288,7 @@ def test_heat_3d_iterate(gpu, oracle, dt):
 
 
 @pytest.mark.parametrize("pad", ["cond", "halo_out", "halo_in"])
-@pytest.mark.parametrize("bname", ["remove", "remove7", "wrap"])
+@pytest.mark.parametrize("bname", ["remove", "remove7", "wrap", "reflect"])
 @pytest.mark.parametrize("shape", [(70, 17, 35), (3, 3, 3), (130, 40, 33)])
 def test_fused_3d_iterate(gpu, oracle, shape, bname, pad):
     """Iterations of 3-D radius-1 stencils advance two steps per launch in the
@@ -312,8 +312,8 @@ def test_fused_3d_iterate(gpu, oracle, shape, bname, pad):
 
 
 @pytest.mark.parametrize("pad", ["cond", "halo_out", "halo_in"])
-@pytest.mark.parametrize("bname", ["remove", "remove7", "wrap"])
-@pytest.mark.parametrize("shape", [(300, 37), (3, 3), (530, 70), (17, 5)])
+@pytest.mark.parametrize("bname", ["remove", "remove7", "wrap", "reflect"])
+@pytest.mark.parametrize("shape", [(300, 37), (3, 3), (530, 70), (17, 5), (224, 9), (225, 8)])
 def test_fused_2d_iterate(gpu, oracle, shape, bname, pad):
     """Iterations of 2-D radius-1 stencils on aligned rows advance two steps
     per launch in the steady state; both buffers (halos included) must equal
