@@ -25,6 +25,8 @@
 // mirror (src/array.jl:154-166) before generation t+2 reads them.
 #pragma once
 
+#include <type_traits>
+
 #include "tile_kernel.hpp"
 #include "zmarch2_kernel.hpp"
 
@@ -60,13 +62,22 @@ __device__ __forceinline__ void ym2_fold_terms(typename AccOf<S, OPK>::type& acc
   }
 }
 
-// The v-th element of a lane's vector for a wave-uniform index v.
+// c ? x : y on values that stay values: a select between two elements of one
+// register array would otherwise become a load through a selected address,
+// for which the compiler moves the array to LDS.
+template <typename Val>
+__device__ __forceinline__ Val select_val(bool c, Val x, Val y) {
+  STH_OPAQUE(x);
+  STH_OPAQUE(y);
+  return c ? x : y;
+}
+
+// element `k` of q for the one bit k set in `bits` (bits is a run-time mask)
 template <int V, typename Val>
-__device__ __forceinline__ Val pick(const Val (&q)[V], int v) {
+__device__ __forceinline__ Val pick(const Val (&q)[V], uint32_t bits) {
   Val r = q[0];
 #pragma unroll
-  for (int k = 1; k < V; ++k)
-    if (k == v) r = q[k];
+  for (int k = 1; k < V; ++k) r = select_val(((bits >> k) & 1u) != 0, q[k], r);
   return r;
 }
 
@@ -167,107 +178,127 @@ __global__ __launch_bounds__(256) void ymarch2_kernel(MapParams p) {
   // x = n0, take the values of the cells they mirror, x = 1 and x = n0 - 2 --
   // the reference reads those cells there (src/array.jl:154-166), and a fold
   // over the mirrored neighbourhood would sum in the reverse order.
+  // The lanes and elements concerned are fixed per wave, so the fix-up is
+  // selects on per-lane masks, with no branch in the row loop.
+  const int32_t pr = n0 - (X0 - V);  // x = n0 in this wave's cells (uniform)
+  const bool rim_in = pr >= 0 && pr < 64 * V;
+  const int32_t pr_c = rim_in ? pr : 2;  // any in-range position when unused
+  const int rim_lane = pr_c / V, rim_elem = pr_c % V;
+  const int mir_lane = (pr_c - 2) / V, mir_elem = (pr_c - 2) % V;  // x = n0 - 2
+  const bool left_fix = X0 == 0 && lane == 0;
+  const bool right_fix = rim_in && lane == rim_lane;
+  const bool from_prev = mir_lane != rim_lane;
+  // Element choices as bit masks the compiler cannot see through: a runtime
+  // element index would make it index the register arrays, which it then
+  // moves to LDS.
+  uint32_t mir_bits = 1u << mir_elem;
+  uint32_t rim_bits = right_fix ? 1u << rim_elem : 0u;
+  STH_OPAQUE(mir_bits);
+  STH_OPAQUE(rim_bits);
   auto reflect_columns = [&](Val (&h)[V]) {
-    if (X0 == 0) {  // wave-uniform
-      const Val from_right = lane_shift<1>(h[1]);  // lane l: lane l+1's element 1
-      if (lane == 0) h[V - 1] = from_right;
-    }
-    const int32_t pr = n0 - (X0 - V);  // x = n0 in this wave's cells (uniform)
-    if (pr >= 0 && pr < 64 * V) {
-      const int lr = pr / V, ir = pr % V;
-      const int pm = pr - 2, lm = pm / V, im = pm % V;  // x = n0 - 2 (pm >= 0)
-      const Val same = pick<V>(h, im);
-      const Val prev = lane_shift<-1>(same);  // lane l: lane l-1's element im
-      const Val v = lm == lr ? same : prev;
-      if (lane == lr) {
+    const Val from_right = lane_shift<1>(h[1]);  // lane l: lane l+1's element 1 (x = 1 for lane 0)
+    const Val same = pick<V>(h, mir_bits);
+    const Val prev = lane_shift<-1>(same);  // lane l: lane l-1's element mir_elem
+    const Val mirrored = from_prev ? prev : same;
 #pragma unroll
-        for (int k = 0; k < V; ++k)
-          if (k == ir) h[k] = v;
-      }
+    for (int k = 0; k < V; ++k) {
+      if (k == V - 1) h[k] = left_fix ? from_right : h[k];
+      h[k] = ((rim_bits >> k) & 1u) ? mirrored : h[k];
     }
   };
 
-  const int nit = (ye - yb) + 2;  // generation t+1 rows yb-1 .. ye; generation-t rows 0 .. nit+1
-#pragma unroll
-  for (int i = 0; i < QD - 1; ++i)
-    if (i <= nit + 1) load_row(i, g0[i]);
-  edges(g0[0], e0[0]);
-  edges(g0[1], e0[1]);
-  for (int j0 = 0; j0 < nit; j0 += QD) {
-#pragma unroll
-    for (int u = 0; u < QD; ++u) {
-      const int j = j0 + u;
-      if (j >= nit) break;
-      // row j + QD - 1 into the slot row j - 1 held
-      if (j + QD - 1 <= nit + 1) load_row(j + QD - 1, g0[(u + QD - 1) % QD]);
-      edges(g0[(u + 2) % QD], e0[(u + 2) % QD]);
-      // generation t+1 on row yb - 1 + j
-      {
-        const auto& q0 = g0[u % QD];
-        const auto& q1 = g0[(u + 1) % QD];
-        const auto& q2 = g0[(u + 2) % QD];
-        const Val lm[3] = {e0[u % QD][0], e0[(u + 1) % QD][0], e0[(u + 2) % QD][0]};
-        const Val rp[3] = {e0[u % QD][1], e0[(u + 1) % QD][1], e0[(u + 2) % QD][1]};
-        auto& h = g1[u & 3];
-#pragma unroll
-        for (int i = 0; i < V; ++i) {
-          A acc = A(0);
-          uint32_t cnt = 0;
-          ym2_fold_terms<S, OPK, Sh, V, 0>(acc, cnt, q0, q1, q2, lm, rp, i, p);
-          h[i] = stage<S>(narrow<O>(fold_finish<S, OPK>(acc, cnt, q1[i], Sh::L, p)));
-        }
-        if constexpr (REMOVE) {
-          const int32_t y = yb - 1 + j;
-          const bool yout = y < 0 || y >= n1;
-#pragma unroll
-          for (int i = 0; i < V; ++i)
-            if (xout[i] || yout) h[i] = pv;
-        }
-        if constexpr (REFLECT) reflect_columns(h);
-        edges(h, e1[u & 3]);
-      }
-      // generation t+2 on row yb - 2 + j
-      if (j >= 2) {
-        const int64_t y = yb - 2 + j;
-        const auto& q1 = g1[(u + 3) & 3];
-        RawVec<ORaw, V> out;
-        auto fold_row = [&](const Val (&q0)[V], const Val (&q2)[V], const Val (&lm)[3], const Val (&rp)[3]) {
-#pragma unroll
+  // Reflect fix-ups run only in waves that touch the array's boundary; the
+  // others take the same branch-free body as Wrap (branches inside the row
+  // loop make the compiler wait for the prefetched rows early).
+  const bool edge_wave = REFLECT && (X0 == 0 || rim_in || yb == 0 || ye == n1);
+  auto march = [&](auto edge_tag) {
+    constexpr bool EDGE = decltype(edge_tag)::value;
+    const int nit = (ye - yb) + 2;  // generation t+1 rows yb-1 .. ye; generation-t rows 0 .. nit+1
+  #pragma unroll
+    for (int i = 0; i < QD - 1; ++i)
+      if (i <= nit + 1) load_row(i, g0[i]);
+    edges(g0[0], e0[0]);
+    edges(g0[1], e0[1]);
+    for (int j0 = 0; j0 < nit; j0 += QD) {
+  #pragma unroll
+      for (int u = 0; u < QD; ++u) {
+        const int j = j0 + u;
+        if (j >= nit) break;
+        // row j + QD - 1 into the slot row j - 1 held
+        if (j + QD - 1 <= nit + 1) load_row(j + QD - 1, g0[(u + QD - 1) % QD]);
+        edges(g0[(u + 2) % QD], e0[(u + 2) % QD]);
+        // generation t+1 on row yb - 1 + j
+        {
+          const auto& q0 = g0[u % QD];
+          const auto& q1 = g0[(u + 1) % QD];
+          const auto& q2 = g0[(u + 2) % QD];
+          const Val lm[3] = {e0[u % QD][0], e0[(u + 1) % QD][0], e0[(u + 2) % QD][0]};
+          const Val rp[3] = {e0[u % QD][1], e0[(u + 1) % QD][1], e0[(u + 2) % QD][1]};
+          auto& h = g1[u & 3];
+  #pragma unroll
           for (int i = 0; i < V; ++i) {
             A acc = A(0);
             uint32_t cnt = 0;
             ym2_fold_terms<S, OPK, Sh, V, 0>(acc, cnt, q0, q1, q2, lm, rp, i, p);
-            out.e[i] = narrow<O>(fold_finish<S, OPK>(acc, cnt, q1[i], Sh::L, p));
+            h[i] = stage<S>(narrow<O>(fold_finish<S, OPK>(acc, cnt, q1[i], Sh::L, p)));
           }
-        };
-        const Val lm[3] = {e1[(u + 2) & 3][0], e1[(u + 3) & 3][0], e1[u & 3][0]};
-        const Val rp[3] = {e1[(u + 2) & 3][1], e1[(u + 3) & 3][1], e1[u & 3][1]};
-        // Reflect: rows y = -1 and y = n1 read rows 1 and n1 - 2 (a
-        // wave-uniform branch taken on the array's first and last rows only)
-        if (REFLECT && (y == 0 || y == n1 - 1)) {
-          const bool top = y == 0, bottom = y == n1 - 1;
-          Val q0[V], q2[V];
-#pragma unroll
-          for (int i = 0; i < V; ++i) {
-            q0[i] = top ? g1[u & 3][i] : g1[(u + 2) & 3][i];
-            q2[i] = bottom ? g1[(u + 2) & 3][i] : g1[u & 3][i];
+          if constexpr (REMOVE) {
+            const int32_t y = yb - 1 + j;
+            const bool yout = y < 0 || y >= n1;
+  #pragma unroll
+            for (int i = 0; i < V; ++i)
+              if (xout[i] || yout) h[i] = pv;
           }
-          const Val lr[3] = {top ? lm[2] : lm[0], lm[1], bottom ? lm[0] : lm[2]};
-          const Val rr[3] = {top ? rp[2] : rp[0], rp[1], bottom ? rp[0] : rp[2]};
-          fold_row(q0, q2, lr, rr);
-        } else {
-          fold_row(g1[(u + 2) & 3], g1[u & 3], lm, rp);
+          if constexpr (EDGE) reflect_columns(h);
+          edges(h, e1[u & 3]);
         }
-        const uint32_t yoff = static_cast<uint32_t>(y * p.dst.s1 * es);
-        if (store_vec) {
-          buf_store<RawVec<ORaw, V>>(out, dst, xs[0], yoff);
-        } else {
-#pragma unroll
-          for (int i = 0; i < V; ++i) buf_store<ORaw>(out.e[i], dst, xs[i], yoff);
+        // generation t+2 on row yb - 2 + j
+        if (j >= 2) {
+          const int64_t y = yb - 2 + j;
+          const auto& q1 = g1[(u + 3) & 3];
+          RawVec<ORaw, V> out;
+          auto fold_row = [&](const Val (&q0)[V], const Val (&q2)[V], const Val (&lm)[3], const Val (&rp)[3]) {
+  #pragma unroll
+            for (int i = 0; i < V; ++i) {
+              A acc = A(0);
+              uint32_t cnt = 0;
+              ym2_fold_terms<S, OPK, Sh, V, 0>(acc, cnt, q0, q1, q2, lm, rp, i, p);
+              out.e[i] = narrow<O>(fold_finish<S, OPK>(acc, cnt, q1[i], Sh::L, p));
+            }
+          };
+          const Val lm[3] = {e1[(u + 2) & 3][0], e1[(u + 3) & 3][0], e1[u & 3][0]};
+          const Val rp[3] = {e1[(u + 2) & 3][1], e1[(u + 3) & 3][1], e1[u & 3][1]};
+          // Reflect: rows y = -1 and y = n1 read rows 1 and n1 - 2 (uniform
+          // selects, effective on the array's first and last rows only)
+          if constexpr (EDGE) {
+            const bool top = y == 0, bottom = y == n1 - 1;
+            Val q0[V], q2[V];
+  #pragma unroll
+            for (int i = 0; i < V; ++i) {
+              q0[i] = select_val(top, g1[u & 3][i], g1[(u + 2) & 3][i]);
+              q2[i] = select_val(bottom, g1[(u + 2) & 3][i], g1[u & 3][i]);
+            }
+            const Val lr[3] = {select_val(top, lm[2], lm[0]), lm[1], select_val(bottom, lm[0], lm[2])};
+            const Val rr[3] = {select_val(top, rp[2], rp[0]), rp[1], select_val(bottom, rp[0], rp[2])};
+            fold_row(q0, q2, lr, rr);
+          } else {
+            fold_row(g1[(u + 2) & 3], g1[u & 3], lm, rp);
+          }
+          const uint32_t yoff = static_cast<uint32_t>(y * p.dst.s1 * es);
+          if (store_vec) {
+            buf_store<RawVec<ORaw, V>>(out, dst, xs[0], yoff);
+          } else {
+  #pragma unroll
+            for (int i = 0; i < V; ++i) buf_store<ORaw>(out.e[i], dst, xs[i], yoff);
+          }
         }
       }
     }
-  }
+  };
+  if (edge_wave)
+    march(std::true_type{});
+  else
+    march(std::false_type{});
 }
 
 // Grid of ymarch2_kernel for an n0 x n1 array: 56 V-column tiles x groups of
