@@ -56,7 +56,8 @@ def kernel_launch_us(sh, torch, S, rule, launches: int = 200):
     """Average duration of the single-generation kernel: graph-replayed steps
     without temporal blocking, between HIP events on the launch stream."""
     stream = torch.cuda.current_stream()
-    S = sh.mapstencil_steps(rule, S, 8, graph=True, flags=sh._abi.ASYNC | sh._abi.NO_FUSE)
+    for _ in range(1 if launches % 2 == 0 else 2):  # capture this schedule's graphs before timing
+        S = sh.mapstencil_steps(rule, S, launches, graph=True, flags=sh._abi.ASYNC | sh._abi.NO_FUSE)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     e0.record(stream)
@@ -64,6 +65,28 @@ def kernel_launch_us(sh, torch, S, rule, launches: int = 200):
     e1.record(stream)
     torch.cuda.synchronize()
     return S, e0.elapsed_time(e1) / launches * 1e3
+
+
+def life2_launch_us(sh, torch, S, rule, m1: int = 2, m2: int = 17):
+    """Average duration of one life2_kernel launch (two generations).  A call
+    of 3 + 64 m generations runs one eager step, m replays of the captured
+    16-unit graph (32 life2_kernel launches each) and the same 2-step tail
+    for every m, so the difference of two such calls, timed between HIP
+    events on the launch stream, is (m2 - m1) x 32 launches of the kernel
+    alone.  Both calls' graphs are captured before timing."""
+    stream = torch.cuda.current_stream()
+    for m in (m1, m2, m1, m2):  # capture both keys at both buffer parities
+        S = sh.mapstencil_steps(rule, S, 3 + 64 * m, graph=True, flags=sh._abi.ASYNC)
+    t = {}
+    for m in (m1, m2):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        S = sh.mapstencil_steps(rule, S, 3 + 64 * m, graph=True, flags=sh._abi.ASYNC)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        t[m] = e0.elapsed_time(e1) * 1e3  # us
+    return S, (t[m2] - t[m1]) / ((m2 - m1) * 32)
 
 
 def profiled_traffic(name: str):
@@ -489,14 +512,18 @@ def main():
 
     S = device_switching(A, sh, torch)
     S, _, _ = time_steps(sh, torch, S, rule, args.warmup, graph=True)
+    # The iteration engine captures a hipGraph per schedule the first time it
+    # sees it; run the timed call's schedule once untimed (at both buffer
+    # parities when the step count is odd) so the timed steps only replay.
+    for _ in range(1 if args.steps % 2 == 0 else 2):
+        S, _, _ = time_steps(sh, torch, S, rule, args.steps, graph=True)
     S, wall, ev = time_steps(sh, torch, S, rule, args.steps, graph=True)
     bytes_per_step = 2 * cells  # 1 B read + 1 B written per interior cell and generation
+    gen_us = ev / args.steps * 1e6
     # The graph-replayed steady steps run life2_kernel: two generations per
     # launch (temporal blocking), so a launch's algorithmic bytes cover two
-    # generations; its duration is bounded by twice the per-generation time
-    # measured between HIP events on the launch stream over the timed steps.
-    gen_us = ev / args.steps * 1e6
-    launch_us = 2 * gen_us
+    # generations.  Its duration is timed on its own (life2_launch_us).
+    S, launch_us = life2_launch_us(sh, torch, S, rule)
     bytes_per_launch = 2 * bytes_per_step
     achieved = bytes_per_launch / (launch_us * 1e-6) / 1e9
     S, plain_launch_us = kernel_launch_us(sh, torch, S, rule)

@@ -108,6 +108,19 @@ typedef enum {
 
 /* sth_op.flags */
 #define STH_OPF_ACC_F32 1u /* BF16/F16 storage: widen to f32, round once at store */
+/*
+ * STH_OPF_INT_WRAP: the caller states that, on its side, `sum` over a Stencil
+ * of an integer element type T folds with T's own `+` (wrapping on
+ * overflow), and that `mean` over it is that sum converted to Float64 and
+ * divided by L.  The reference leaves both to StaticArrays (a Stencil is a
+ * StaticVector, src/stencil.jl:17), which it does not pin, and Base's `sum`
+ * would widen small integers instead.  Without this flag the library
+ * declines (STH_ENOTSUP) `sum` over Int8..UInt32 and `mean` over every
+ * integer type except Bool whenever L >= 2, i.e. wherever the two readings
+ * can differ; the Julia shim sets it after checking its own StaticArrays on
+ * one overflowing SVector.
+ */
+#define STH_OPF_INT_WRAP 2u
 
 typedef struct sth_array {
   void* data;         /* device pointer to the parent's first element */

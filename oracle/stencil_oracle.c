@@ -19,8 +19,16 @@
  * Reductions (the fixed set) fold left over the offsets in offset order:
  *   sum      StaticArrays `sum` (external, StaticArrays 1.x): left fold of `+`
  *            in the element type (integers wrap); Bool sums to Int64.
- *   mean     Statistics `mean` (external, Statistics 1.x): every element
- *            converted to float(T), left-fold sum, then `/ L`.
+ *   mean     `mean` over the Stencil, a StaticVector: floats fold in T and
+ *            divide by L (Statistics' `mean` does the same for floats);
+ *            integers take the element-type `sum` above (wrapping; Bool
+ *            counts in Int64) converted to Float64, then `/ L` -- what
+ *            StaticArrays' `mean(a) = sum(a) / length(a)` gives.  Which of
+ *            StaticArrays' or Statistics' generic method (every element
+ *            promoted to Float64 first) a Stencil reaches is not pinned by
+ *            anything the reference holds; the library accelerates integer
+ *            sum/mean where the two can differ only when its caller says
+ *            which (STH_OPF_INT_WRAP).  PARITY UNPINNED for those cases.
  *   min/max  StaticArrays `minimum`/`maximum`: left fold of Julia `min`/`max`
  *            (NaN-propagating, -0.0 < +0.0).
  *   kernelproduct  src/stencils/kernel.jl:37-43: acc = zero(T);
@@ -183,6 +191,24 @@ static uint64_t load_uint(const void* p, int dt) {
     case STH_U64: return *(const uint64_t*)p;
   }
   return 0;
+}
+/* A 64-bit two's-complement accumulator reduced to the width of `dt`, as a
+ * fold with that type's own `+` leaves it. */
+static int64_t wrap_sint(uint64_t u, int dt) {
+  switch (dt) {
+    case STH_I8: return (int8_t)u;
+    case STH_I16: return (int16_t)u;
+    case STH_I32: return (int32_t)u;
+  }
+  return (int64_t)u;
+}
+static uint64_t wrap_uint(uint64_t u, int dt) {
+  switch (dt) {
+    case STH_U8: return (uint8_t)u;
+    case STH_U16: return (uint16_t)u;
+    case STH_U32: return (uint32_t)u;
+  }
+  return u;
 }
 
 /* Julia max/min on IEEE floats: NaN if either is NaN, -0.0 < +0.0. */
@@ -361,17 +387,24 @@ static void cell(const job_t* j, int64_t i0, int64_t i1, int64_t i2) {
   }
 
   /* integers (and Bool) */
+  int sgn = j->cat == CAT_SINT;
   if (kind == STH_OP_MEAN) {
-    double acc = 0.0;
+    /* sum(a) / length(a) over the StaticVector: the sum folds with the
+     * element type's wrapping `+` (Bool + Bool is an Int), then Float64 / L */
+    uint64_t uacc = 0;
     for (int k = 0; k < L; ++k) {
       const char* p = fetch(j, i0 + j->offs[3 * k], i1 + j->offs[3 * k + 1], i2 + j->offs[3 * k + 2]);
-      double v = load_f64(p ? p : (const char*)j->padval, dt);
-      acc = k == 0 ? v : acc + v;
+      const char* q = p ? p : (const char*)j->padval;
+      uint64_t uv = sgn ? (uint64_t)load_sint(q, dt) : load_uint(q, dt);
+      uacc = k == 0 ? uv : uacc + uv;
     }
-    *(double*)out = acc / (double)L;
+    double s;
+    if (dt == STH_BOOL) s = (double)(int64_t)uacc;
+    else if (sgn) s = (double)wrap_sint(uacc, dt);
+    else s = (double)wrap_uint(uacc, dt);
+    *(double*)out = s / (double)L;
     return;
   }
-  int sgn = j->cat == CAT_SINT;
   uint64_t uacc = 0;
   int64_t sacc = 0;
   for (int k = 0; k < L; ++k) {

@@ -237,6 +237,16 @@ int make_plan(const sth_array* src, const sth_array* dst, const sth_stencil* st,
     return fail(STH_EINVAL, "kernelproduct needs a Kernel stencil (weights)");
   const int odt = out_dtype(sdt, *op, wdt);
   if (odt < 0) return fail(STH_ENOTSUP, "op %d over dtype %d (weights %d) is not accelerated", op->kind, sdt, wdt);
+  // Integer sum / mean whose result depends on how the caller's `sum` folds
+  // a StaticVector (wrapping in T or widening): only with STH_OPF_INT_WRAP.
+  if (st->length >= 2 && !(op->flags & STH_OPF_INT_WRAP) && !is_float(sdt) && sdt != STH_BOOL) {
+    const bool narrow_int = dtype_size(sdt) < 8;
+    if ((op->kind == STH_OP_SUM && narrow_int) || op->kind == STH_OP_MEAN)
+      return fail(STH_ENOTSUP,
+                  "%s over integer dtype %d with %d offsets: overflow semantics unpinned (pass STH_OPF_INT_WRAP "
+                  "if the caller's sum wraps in the element type)",
+                  op->kind == STH_OP_SUM ? "sum" : "mean", sdt, st->length);
+  }
   if (odt != dst->dtype)
     return fail(STH_EINVAL, "dest dtype %d does not match the result type %d", dst->dtype, odt);
 

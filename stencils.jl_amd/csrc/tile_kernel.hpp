@@ -29,7 +29,7 @@ template <int OPK> struct AccOf<f16_t, OPK> { using type = float; };
 #define STH_INT_ACC(T)                                                        \
   template <> struct AccOf<T, OP_SUM> { using type = Elem<T>::uacc; };        \
   template <> struct AccOf<T, OP_KP> { using type = Elem<T>::uacc; };         \
-  template <> struct AccOf<T, OP_MEAN> { using type = double; };              \
+  template <> struct AccOf<T, OP_MEAN> { using type = Elem<T>::uacc; };       \
   template <> struct AccOf<T, OP_LIFE> { using type = uint32_t; };
 STH_INT_ACC(bool)
 STH_INT_ACC(uint8_t)
@@ -41,6 +41,25 @@ STH_INT_ACC(int32_t)
 STH_INT_ACC(uint64_t)
 STH_INT_ACC(int64_t)
 #undef STH_INT_ACC
+
+// Type fold_finish returns: the accumulator's, except for an integer mean,
+// which folds in the (wrapping) integer accumulator and divides in Float64.
+template <typename S, int OPK>
+struct FinOf {
+  using type = typename AccOf<S, OPK>::type;
+};
+#define STH_INT_FIN(T) \
+  template <> struct FinOf<T, OP_MEAN> { using type = double; };
+STH_INT_FIN(bool)
+STH_INT_FIN(uint8_t)
+STH_INT_FIN(int8_t)
+STH_INT_FIN(uint16_t)
+STH_INT_FIN(int16_t)
+STH_INT_FIN(uint32_t)
+STH_INT_FIN(int32_t)
+STH_INT_FIN(uint64_t)
+STH_INT_FIN(int64_t)
+#undef STH_INT_FIN
 
 // Storage type of the result (the reference's _return_type).
 template <typename S, int OPK>
@@ -139,14 +158,21 @@ __device__ __forceinline__ void fold_step(typename AccOf<S, OPK>::type& acc, uin
 }
 
 // Finish the fold: mean divides by L, the Laplacian and Life rules combine
-// with the centre value `cv`.
+// with the centre value `cv`.  An integer mean is the sum folded in the
+// element type (wrapping; Bool counts in Int64) converted to Float64 and
+// divided by L, which is `sum(a) / length(a)` over a StaticVector.
 template <typename S, int OPK, class P>
-__device__ __forceinline__ typename AccOf<S, OPK>::type fold_finish(typename AccOf<S, OPK>::type acc,
+__device__ __forceinline__ typename FinOf<S, OPK>::type fold_finish(typename AccOf<S, OPK>::type acc,
                                                                     uint32_t cnt,
                                                                     typename Staged<S>::type cv,
                                                                     int L, const P& p) {
   using A = typename AccOf<S, OPK>::type;
-  if constexpr (OPK == OP_MEAN) {
+  if constexpr (OPK == OP_MEAN && !Elem<S>::is_float) {
+    double s;
+    if constexpr (__is_same(S, bool)) s = static_cast<double>(static_cast<int64_t>(acc));
+    else s = static_cast<double>(static_cast<typename Elem<S>::raw>(acc));
+    return s / static_cast<double>(L);
+  } else if constexpr (OPK == OP_MEAN) {
     acc = acc / static_cast<A>(L);
   } else if constexpr (OPK == OP_LAP) {
     A c;
@@ -166,7 +192,7 @@ __device__ __forceinline__ typename AccOf<S, OPK>::type fold_finish(typename Acc
 
 // One output value from the staged box; `c` is the LDS index of the centre.
 template <typename S, int OPK, class Sh, int W, int HW, class P>
-__device__ __forceinline__ typename AccOf<S, OPK>::type reduce_cell(
+__device__ __forceinline__ typename FinOf<S, OPK>::type reduce_cell(
     const typename Staged<S>::type* __restrict__ tile, int c, const P& p,
     int wdyn, int hwdyn) {
   using A = typename AccOf<S, OPK>::type;

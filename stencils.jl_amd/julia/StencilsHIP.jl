@@ -26,6 +26,8 @@ using Stencils: AbstractStencilArray, StencilArray, SwitchingStencilArray, Stenc
     kernelproduct, offsets, radius, padding, boundary, stencil, source, dest, switch,
     neighbors, center, kernel
 using AMDGPU: ROCArray
+import AMDGPU
+using StaticArrays: SVector
 
 export LifeRule, Laplacian, mapstencil_steps!, mapstencil_batch!, sth_available
 export Server, submit!, start!, completions!, drain!, close!
@@ -37,6 +39,7 @@ const libsth = get(ENV, "STENCILS_HIP_LIB",
 const STH_OK, STH_EDEVICE, STH_ENOMEM, STH_EINVAL, STH_ENOTSUP = 0, 5, 12, 22, 95
 const STH_ASYNC, STH_GRAPH = UInt32(1), UInt32(2)
 const STH_OPF_ACC_F32 = UInt32(1)
+const STH_OPF_INT_WRAP = UInt32(2)
 
 const DTYPES = Dict{DataType,Int32}(
     Bool => 0, UInt8 => 1, Int8 => 2, UInt16 => 3, Int16 => 4, UInt32 => 5, Int32 => 6,
@@ -159,8 +162,8 @@ function (l::Laplacian)(hood)
 end
 
 # --------------------------------------------------------- op descriptors
-_op(::typeof(sum), T) = SthOp(0, _accflag(T), 0.0, 0, 0)
-_op(::typeof(mean), T) = SthOp(1, _accflag(T), 0.0, 0, 0)
+_op(::typeof(sum), T) = SthOp(0, _accflag(T) | _intflag(T), 0.0, 0, 0)
+_op(::typeof(mean), T) = SthOp(1, _accflag(T) | _intflag(T), 0.0, 0, 0)
 _op(::typeof(minimum), T) = SthOp(2, _accflag(T), 0.0, 0, 0)
 _op(::typeof(maximum), T) = SthOp(3, _accflag(T), 0.0, 0, 0)
 _op(::typeof(kernelproduct), T) = SthOp(4, _accflag(T), 0.0, 0, 0)
@@ -169,6 +172,29 @@ _op(l::Laplacian, T) = SthOp(6, _accflag(T), l.alpha, 0, 0)
 _op(f, T) = nothing
 _accflag(::Type{Float16}) = STH_OPF_ACC_F32
 _accflag(::Type) = UInt32(0)
+
+# How `sum` and `mean` fold a Stencil (a StaticVector) of an integer type is
+# decided by the installed StaticArrays / Statistics, which the reference
+# does not pin.  The library accelerates integer sum / mean where the
+# readings differ only when told that the sum wraps in the element type and
+# the mean is that sum / L; this checks it once per type on an SVector whose
+# sum overflows.  Otherwise the call falls back to the reference method.
+const _INTWRAP = Dict{DataType,Bool}()
+function _intwraps(::Type{T}) where {T<:Integer}
+    get!(_INTWRAP, T) do
+        v = SVector{2,T}(typemax(T), one(T))
+        w = typemax(T) + one(T)   # wraps in T
+        s = sum(v)
+        s isa T && s == w && mean(v) == Float64(w) / 2
+    end
+end
+_intflag(::Type{Bool}) = UInt32(0)
+_intflag(::Type{T}) where {T<:Integer} = _intwraps(T) ? STH_OPF_INT_WRAP : UInt32(0)
+_intflag(::Type) = UInt32(0)
+
+# The task-local AMDGPU stream: library work is ordered with the caller's
+# kernels and copies on it (AMDGPU.HIP.HIPStream wraps the hipStream_t).
+_stream() = Ptr{Cvoid}(AMDGPU.stream().stream)
 
 const AccelOp = Union{typeof(sum),typeof(mean),typeof(minimum),typeof(maximum),
                       typeof(kernelproduct),LifeRule,Laplacian}
@@ -236,7 +262,7 @@ function _map(f, dest_parent::ROCArray, dest_halo::Int, src::StencilArray{R,T}) 
                       Ptr{Cvoid}, Int32, Ref{SthOp}, UInt32, Ptr{Cvoid}),
                      _context(), sa, da, sd, _bc(boundary(src)),
                      Base.unsafe_convert(Ptr{Cvoid}, pv), _pad(padding(src)), op,
-                     UInt32(0), C_NULL)
+                     UInt32(0), _stream())
     end
 end
 
@@ -288,7 +314,7 @@ function mapstencil_steps!(f::AccelOp, A::SwitchingStencilArray{R,T,N,<:ROCArray
                Int32, Ref{SthOp}, Int64, UInt32, Ref{Int32}, Ptr{Cvoid}),
               _context(), a, b, sd, _bc(boundary(A)), Base.unsafe_convert(Ptr{Cvoid}, pv),
               _pad(padding(A)), op, Int64(steps),
-              graph ? STH_GRAPH : UInt32(0), cur, C_NULL)
+              graph ? STH_GRAPH : UInt32(0), cur, _stream())
     end
     st == STH_ENOTSUP && return _fallback_steps(f, A, steps)
     st == STH_OK || _throw(st)
@@ -337,7 +363,7 @@ function mapstencil_batch!(calls::AbstractVector)
     end
     st = GC.@preserve keep jobs ccall((:sth_map_batch, libsth), Cint,
                                       (Ptr{Cvoid}, Int32, Ptr{SthJob}, UInt32, Ptr{Cvoid}),
-                                      _context(), Int32(length(jobs)), jobs, UInt32(0), C_NULL)
+                                      _context(), Int32(length(jobs)), jobs, UInt32(0), _stream())
     if st == STH_ENOTSUP
         foreach(c -> Stencils.gatherstencil!(c...), calls)
     elseif st != STH_OK
@@ -367,7 +393,7 @@ function Server(hbm_budget::Integer; round_steps::Integer=8, serial::Bool=false)
     cfg = Ref(SthServerConfig(Csize_t(hbm_budget), 0, Int32(round_steps), serial ? UInt32(1) : UInt32(0), 0))
     h = Ref{Ptr{Cvoid}}(C_NULL)
     st = ccall((:sth_server_create, libsth), Cint,
-               (Ptr{Cvoid}, Ref{SthServerConfig}, Ptr{Cvoid}, Ref{Ptr{Cvoid}}), _context(), cfg, C_NULL, h)
+               (Ptr{Cvoid}, Ref{SthServerConfig}, Ptr{Cvoid}, Ref{Ptr{Cvoid}}), _context(), cfg, _stream(), h)
     st == STH_OK || _throw(st)
     srv = Server(h[], Dict{UInt64,Any}(), UInt64(1))
     finalizer(close!, srv)

@@ -37,6 +37,7 @@ CONDITIONAL, HALO_OUT, HALO_IN = range(3)
 OP_SUM, OP_MEAN, OP_MIN, OP_MAX, OP_KERNELPRODUCT, OP_LIFE, OP_LAPLACIAN = range(7)
 OP_NAMES = ["sum", "mean", "minimum", "maximum", "kernelproduct", "life", "laplacian"]
 OPF_ACC_F32 = 1
+OPF_INT_WRAP = 2
 
 # call flags
 ASYNC = 1
@@ -117,6 +118,17 @@ def out_dtype(src_dt: int, opkind: int, flags: int = 0, wdtype: int = -1) -> int
     if opkind == OP_LAPLACIAN:
         return src_dt if src_dt in FLOAT_DTYPES else -1
     return -1
+
+
+def int_fold_declined(src_dt: int, opkind: int, flags: int, length: int) -> bool:
+    """True where the library declines an integer sum / mean because the
+    result depends on how the caller's `sum` folds (wrapping in the element
+    type or widening) and the caller did not say (OPF_INT_WRAP)."""
+    if length < 2 or (flags & OPF_INT_WRAP) or src_dt in FLOAT_DTYPES or src_dt == BOOL:
+        return False
+    if opkind == OP_SUM:
+        return DTYPE_SIZE[src_dt] < 8
+    return opkind == OP_MEAN
 
 
 def make_array(data_ptr: int, dtype: int, shape, strides_elems, halo: int) -> SthArray:

@@ -24,7 +24,7 @@ from .array import (AbstractStencilArray, StencilArray, SwitchingStencilArray,
                     colmajor_copy, colmajor_empty, is_device, sth_dtype_of,
                     torch_dtype, to_numpy)
 from .boundary import Conditional, Halo, Remove, Use
-from .ops import Reduction, Unsupported, resolve
+from .ops import Reduction, Unsupported, integer_fold_wraps, resolve
 from .stencil import Kernel, Stencil
 
 
@@ -107,16 +107,23 @@ def stencil_descriptor(st: Stencil, src_dt: int, call: _Call) -> _abi.SthStencil
 
 
 def _op_flags(src_dt: int) -> int:
-    return _abi.OPF_ACC_F32 if src_dt in (_abi.BF16, _abi.F16) else 0
+    fl = _abi.OPF_ACC_F32 if src_dt in (_abi.BF16, _abi.F16) else 0
+    if integer_fold_wraps() and src_dt not in _abi.FLOAT_DTYPES:
+        fl |= _abi.OPF_INT_WRAP
+    return fl
 
 
 def result_dtype(f, sa: AbstractStencilArray) -> int:
     red = resolve(f)
     dt = sa.sth_dtype()
     _, wdt = _weights(sa.stencil, dt)
-    o = _abi.out_dtype(dt, red.kind, _op_flags(dt), wdt)
+    fl = _op_flags(dt)
+    o = _abi.out_dtype(dt, red.kind, fl, wdt)
     if o < 0:
         raise Unsupported(f"{red!r} over {_abi.DTYPE_NAMES[dt]} is not accelerated")
+    if _abi.int_fold_declined(dt, red.kind, fl, len(sa.stencil)):
+        raise Unsupported(f"{red!r} over {_abi.DTYPE_NAMES[dt]} with {len(sa.stencil)} offsets: integer "
+                          "overflow semantics unpinned (set_integer_fold_wraps)")
     return o
 
 

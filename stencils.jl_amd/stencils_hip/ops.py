@@ -89,6 +89,27 @@ _ALIASES = {
 }
 
 
+_INT_FOLD_WRAPS = False
+
+
+def set_integer_fold_wraps(flag: bool) -> bool:
+    """State how this host's `sum` over a Stencil of an integer type folds:
+    True when it uses the element type's own wrapping `+` (StaticArrays'
+    reduction; `mean` is then that sum / L in Float64).  The reference does
+    not pin it (a Stencil is a StaticVector, src/stencil.jl:17), so integer
+    `sum` over Int8..UInt32 and integer `mean` are accelerated only after a
+    caller has said so; otherwise they raise Unsupported.  Returns the
+    previous setting."""
+    global _INT_FOLD_WRAPS
+    old = _INT_FOLD_WRAPS
+    _INT_FOLD_WRAPS = bool(flag)
+    return old
+
+
+def integer_fold_wraps() -> bool:
+    return _INT_FOLD_WRAPS
+
+
 def resolve(f) -> Reduction:
     """Map a user function onto the accelerated set, or raise Unsupported."""
     if isinstance(f, Reduction):

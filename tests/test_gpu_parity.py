@@ -102,9 +102,18 @@ def test_parity_f32_2d(gpu, oracle, st, bname):
         np.testing.assert_array_equal(g, o)
 
 
+@pytest.fixture
+def int_wrap():
+    """Integer sum / mean with the element type's wrapping fold stated by the
+    caller (the library declines them otherwise)."""
+    old = sh.set_integer_fold_wraps(True)
+    yield
+    sh.set_integer_fold_wraps(old)
+
+
 @pytest.mark.parametrize("dt", [np.bool_, np.uint8, np.int8, np.uint16, np.int16, np.uint32,
                                 np.int32, np.uint64, np.int64], ids=lambda d: np.dtype(d).name)
-def test_parity_integers(gpu, oracle, dt):
+def test_parity_integers(gpu, oracle, int_wrap, dt):
     A = _np_input((53, 38), dt, 3)
     for st in (sh.Window(1), sh.Moore(2), sh.Circle(2), sh.Positional((0, 1), (2, -1))):
         for bname in ("remove7", "wrap", "reflect"):
@@ -119,6 +128,34 @@ def test_parity_integers(gpu, oracle, dt):
                     K = sh.Kernel(st, np.arange(1, len(st) + 1).astype(dt))
                     g, o, _, _ = _both(sh.kernelproduct, A, K, bc, PADS[pad], oracle)
                     np.testing.assert_array_equal(g, o)
+
+
+@pytest.mark.parametrize("dt", [np.uint8, np.int8, np.int16, np.uint32, np.int32, np.int64, np.uint64])
+def test_unpinned_integer_folds_are_declined(gpu, dt):
+    """Without the caller's statement of how its `sum` folds, the library
+    itself (not only the Python front end) declines integer sums over
+    sub-64-bit types and integer means; Bool, 64-bit sums, min/max and
+    one-offset stencils stay accelerated."""
+    assert not sh.integer_fold_wraps()
+    A = _np_input((20, 17), dt, 30)
+    big = np.dtype(dt).itemsize == 8
+    for f in (sh.sum, sh.mean):
+        sa = sh.StencilArray(A.copy(order="F"), sh.Moore(2), boundary=sh.Wrap())
+        out = np.zeros(A.shape, dtype=np.float64 if f is sh.mean else dt, order="F")
+        if f is sh.sum and big:
+            assert sh.result_dtype(f, sa) == sh._abi.NUMPY_TO_STH[np.dtype(dt)]
+            sh.mapstencil_(f, out, sa)  # 64-bit sums wrap under either reading
+            continue
+        with pytest.raises(sh.Unsupported):
+            sh.result_dtype(f, sa)  # the front end's check
+        with pytest.raises(sh.Unsupported, match="unpinned"):
+            sh.mapstencil_(f, out, sa)  # straight to the library, no front-end check
+    one = sh.StencilArray(A.copy(order="F"), sh.Positional((1, 0)), boundary=sh.Wrap())
+    assert np.array_equal(sh.mapstencil(sh.sum, one), np.roll(A, -1, axis=0))
+    np.testing.assert_array_equal(sh.mapstencil(sh.mean, one), np.roll(A, -1, axis=0).astype(np.float64))
+    b = sh.StencilArray((A % 2 == 0).copy(order="F"), sh.Moore(2), boundary=sh.Wrap())
+    assert sh.mapstencil(sh.sum, b).dtype == np.int64
+    assert sh.mapstencil(sh.maximum, sh.StencilArray(A.copy(order="F"), sh.Moore(2))).dtype == dt
 
 
 @pytest.mark.parametrize("n", [1, 2, 5, 17, 300, 1000])
@@ -335,44 +372,54 @@ def test_fused_2d_iterate(gpu, oracle, shape, bname, pad):
             np.testing.assert_array_equal(G.dest, O.dest, err_msg=f"{dt} {st} {steps}")
 
 
+STORAGE_BCS = {"wrap": lambda: sh.Wrap(), "reflect": lambda: sh.Reflect(),
+               "remove_nz": lambda: sh.Remove(0.75)}
+
+
+def _storage_iterate_pair(torch, shape, dt, st, f, bc, pad, steps):
+    """Both buffers after `steps` iterations with two-step launches and with
+    one launch per step, from the same initial state."""
+    dev = torch.device("cuda")
+    A = sh.colmajor_empty(shape, dt, dev)
+    A.copy_(torch.rand(shape, device=dev).to(dt))
+    out = []
+    for flags in (0, sh._abi.NO_FUSE):
+        src = sh.colmajor_empty(shape, dt, dev)
+        src.copy_(A)
+        S = sh.SwitchingStencilArray(src, st, boundary=bc, padding=pad)
+        S = sh.mapstencil_steps(f, S, steps, flags=flags)
+        out.append((S.source.clone(), S.dest.clone()))
+    return out
+
+
+@pytest.mark.parametrize("pad", ["cond", "halo_out"])
+@pytest.mark.parametrize("bname", list(STORAGE_BCS))
 @pytest.mark.parametrize("tname", ["bfloat16", "float16"])
-def test_fused_2d_iterate_storage_types(gpu, tname):
-    """16-bit storage in 2-D: two-step launches equal the unfused path."""
+def test_fused_2d_iterate_storage_types(gpu, tname, bname, pad):
+    """16-bit storage in 2-D: two-step launches equal the unfused path, for
+    Wrap, Reflect (mirrored rim fix-ups) and Remove with a nonzero padval.
+    n0 = 56 * 8 + 3 puts the right rim of a Halo{:out} pitched row in the
+    previous wave's vectors."""
     import torch
-    dev = torch.device("cuda")
     dt = getattr(torch, tname)
-    shape = (700, 45)
-    A = sh.colmajor_empty(shape, dt, dev)
-    A.copy_(torch.rand(shape, device=dev).to(dt))
-    out = []
-    for flags in (0, sh._abi.NO_FUSE):
-        src = sh.colmajor_empty(shape, dt, dev)
-        src.copy_(A)
-        S = sh.SwitchingStencilArray(src, sh.Window(1), boundary=sh.Wrap())
-        S = sh.mapstencil_steps(sh.mean, S, 17, flags=flags)
-        out.append((S.source.clone(), S.dest.clone()))
-    assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1])
+    for shape in ((700, 45), (56 * 8 + 3, 37)):
+        out = _storage_iterate_pair(torch, shape, dt, sh.Window(1), sh.mean, STORAGE_BCS[bname](), PADS[pad], 17)
+        assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1]), str(shape)
 
 
+@pytest.mark.parametrize("pad", ["cond", "halo_out"])
+@pytest.mark.parametrize("bname", list(STORAGE_BCS))
 @pytest.mark.parametrize("tdt", ["bfloat16", "float16"])
-def test_fused_3d_iterate_storage_types(gpu, tdt):
+def test_fused_3d_iterate_storage_types(gpu, tdt, bname, pad):
     """16-bit storage: the fused launches round the intermediate step to the
-    storage type, so results equal the unfused step-by-step path bit for bit."""
+    storage type, so results equal the unfused step-by-step path bit for bit
+    (Wrap, Reflect, Remove with a nonzero padval)."""
     import torch
-    dev = torch.device("cuda")
     dt = getattr(torch, tdt)
-    shape = (96, 37, 41)
-    A = sh.colmajor_empty(shape, dt, dev)
-    A.copy_(torch.rand(shape, device=dev).to(dt))
     f = sh.Laplacian(0.15) if tdt == "bfloat16" else sh.mean
-    out = []
-    for flags in (0, sh._abi.NO_FUSE):
-        src = sh.colmajor_empty(shape, dt, dev)
-        src.copy_(A)
-        S = sh.SwitchingStencilArray(src, sh.VonNeumann(1, 3), boundary=sh.Wrap())
-        S = sh.mapstencil_steps(f, S, 17, flags=flags)
-        out.append((S.source.clone(), S.dest.clone()))
-    assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1])
+    for shape in ((96, 37, 41), (63, 20, 19)):
+        out = _storage_iterate_pair(torch, shape, dt, sh.VonNeumann(1, 3), f, STORAGE_BCS[bname](), PADS[pad], 17)
+        assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1]), str(shape)
 
 
 # ------------------------------------------------------------ errors

@@ -9,6 +9,7 @@ import pytest
 import scipy.ndimage as ndi
 
 import stencils_hip as sh
+from stencils_hip import _abi
 
 MODES = {"remove": ("constant", lambda: sh.Remove(0.0)), "wrap": ("wrap", sh.Wrap),
          "reflect": ("mirror", sh.Reflect)}
@@ -210,3 +211,64 @@ def test_integer_sum_wraps_in_element_type(oracle):
     assert s.dtype == np.int64 and (s == 9).all()       # Bool + Bool is Int
     m = oracle.mapstencil(sh.mean, sh.StencilArray(B, sh.Window(1), boundary=sh.Wrap()))
     assert m.dtype == np.float64 and (m == 1.0).all()   # mean of Bool is Float64
+
+
+def test_integer_mean_is_element_type_sum_over_length(oracle):
+    """Integer `mean` = `sum(a) / length(a)` over the StaticVector: the sum
+    folds with the element type's wrapping `+` and is then divided in
+    Float64.  Checked by brute force on tiny grids for every integer type,
+    with values chosen to overflow the narrow types (so widening first, the
+    other plausible reading, gives different numbers)."""
+    rng = np.random.default_rng(40)
+    st = sh.Moore(1)
+    offs = st.offsets()
+    for dt in (np.uint8, np.int8, np.uint16, np.int16, np.uint32, np.int32, np.uint64, np.int64):
+        info = np.iinfo(dt)
+        A = np.asfortranarray(rng.integers(info.max // 2, info.max, (7, 6), dtype=dt))
+        if info.min < 0:
+            A[::2] = (-A[::2] - 1).astype(dt)
+        got = oracle.mapstencil(sh.mean, sh.StencilArray(A.copy(order="F"), st, boundary=sh.Wrap()))
+        exp = np.zeros(A.shape)
+        widened = np.zeros(A.shape)
+        for i in range(A.shape[0]):
+            for j in range(A.shape[1]):
+                vals = [int(A[(i + dx) % A.shape[0], (j + dy) % A.shape[1]]) for dx, dy in offs]
+                s = sum(vals) % (1 << (8 * A.itemsize))        # wrap to the width ...
+                if info.min < 0 and s > info.max:               # ... as two's complement
+                    s -= 1 << (8 * A.itemsize)
+                exp[i, j] = float(s) / len(offs)
+                widened[i, j] = sum(float(v) for v in vals) / len(offs)
+        np.testing.assert_array_equal(got, exp, err_msg=np.dtype(dt).name)
+        assert not np.array_equal(got, widened)   # the test input overflows
+
+
+def test_host_declines_unpinned_integer_folds():
+    """Without OPF_INT_WRAP the front end and the ABI mirror decline integer
+    sum over sub-64-bit types and integer mean (L >= 2); with it, or for
+    Bool, 64-bit sums, min/max and L = 1, they are accepted."""
+    A8 = sh.StencilArray(np.zeros((6, 6), np.int8, order="F"), sh.Moore(2))
+    old = sh.set_integer_fold_wraps(False)
+    try:
+        for f in (sh.sum, sh.mean):
+            with pytest.raises(sh.Unsupported, match="unpinned"):
+                sh.result_dtype(f, A8)
+        assert sh.result_dtype(sh.maximum, A8) == sh._abi.I8
+        A64 = sh.StencilArray(np.zeros((6, 6), np.int64, order="F"), sh.Moore(2))
+        assert sh.result_dtype(sh.sum, A64) == sh._abi.I64
+        with pytest.raises(sh.Unsupported):
+            sh.result_dtype(sh.mean, A64)
+        B = sh.StencilArray(np.zeros((6, 6), np.bool_, order="F"), sh.Moore(2))
+        assert sh.result_dtype(sh.sum, B) == sh._abi.I64 and sh.result_dtype(sh.mean, B) == sh._abi.F64
+        one = sh.StencilArray(np.zeros((6, 6), np.int8, order="F"), sh.Positional((1, 0)))
+        assert sh.result_dtype(sh.sum, one) == sh._abi.I8
+        sh.set_integer_fold_wraps(True)
+        assert sh.result_dtype(sh.sum, A8) == sh._abi.I8 and sh.result_dtype(sh.mean, A8) == sh._abi.F64
+    finally:
+        sh.set_integer_fold_wraps(old)
+    for dt in range(13):
+        for op in range(7):
+            for L in (1, 2, 25):
+                declined = _abi.int_fold_declined(dt, op, 0, L)
+                assert declined == (L >= 2 and dt not in _abi.FLOAT_DTYPES and dt != _abi.BOOL and
+                                    ((op == _abi.OP_SUM and _abi.DTYPE_SIZE[dt] < 8) or op == _abi.OP_MEAN))
+                assert not _abi.int_fold_declined(dt, op, _abi.OPF_INT_WRAP, L)
