@@ -268,6 +268,28 @@ int sth_map_batch(sth_ctx* ctx, int32_t njobs, const sth_job* jobs, uint32_t fla
                   void* stream);
 
 /*
+ * sth_map_batch without allocating: the same calls, kernels and results as
+ * sth_map_batch (each result identical to sth_map on that job alone), but
+ * no device or pinned host memory is allocated, no host buffer is copied to
+ * the device, and the autotuner only reuses decisions already made (it does
+ * not time variants).  The batched launches' job tables live in one fixed
+ * device array of STH_STATIC_BATCH_MAX_JOBS entries (819,200 bytes) per
+ * device, held from library load to process exit; they are written by small
+ * kernels whose arguments carry the jobs.  More than
+ * STH_STATIC_BATCH_MAX_JOBS jobs is STH_EINVAL.
+ * Every call on a device shares that device's array: calls that overlap on
+ * one device (on different streams, or from different threads) race; calls
+ * ordered on one stream do not.
+ * No counterpart in the reference (one launch per mapstencil call,
+ * src/gatherstencil.jl:97-100); for callers that cannot allocate device
+ * memory: code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc.
+ */
+#define STH_STATIC_BATCH_MAX_JOBS 4096
+int sth_map_batch_static(sth_ctx* ctx, int32_t njobs, const sth_job* jobs, uint32_t flags,
+                         void* stream);
+
+/*
  * Serving loop: continuous batching of iterated stencil solves.
  *
  * A request is `steps` calls of mapstencil!(f, A::SwitchingStencilArray)

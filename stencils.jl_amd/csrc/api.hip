@@ -24,6 +24,37 @@
 #include "stencils_hip.h"
 
 namespace sth {
+constexpr int kStaticBatchJobs = STH_STATIC_BATCH_MAX_JOBS;
+}  // namespace sth
+
+// Device memory of sth_map_batch_static: the job tables of its batched
+// launches (819,200 bytes; every device the library runs on holds one copy
+// for the life of the process).  Host code never uses this name as an
+// address: sth_ctx_create looks up each device's copy with
+// hipGetSymbolAddress and keeps that pointer in the context.
+static __device__ __align__(256) sth::BatchJob scratch[sth::kStaticBatchJobs];
+
+namespace sth {
+
+// Jobs per fill_jobs_kernel launch: they travel in the kernel arguments
+// (3,200 bytes of the 4 KiB argument space).
+constexpr int kFillJobs = 16;
+struct JobChunk {
+  BatchJob j[kFillJobs];
+};
+
+// Writes `n` jobs of a chunk into a device job table with vector stores.
+__global__ __launch_bounds__(256) void fill_jobs_kernel(BatchJob* __restrict__ dst, JobChunk c, int n) {
+  static_assert(sizeof(BatchJob) % 4 == 0, "jobs copy as 32-bit words");
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(&c);
+  uint32_t* d = reinterpret_cast<uint32_t*>(dst);
+  const int words = n * static_cast<int>(sizeof(BatchJob) / 4);
+  for (int i = static_cast<int>(threadIdx.x); i < words; i += 256) d[i] = src[i];
+}
+
+void launch_fill_jobs(BatchJob* dst, const JobChunk& c, int n, hipStream_t s) {
+  hipLaunchKernelGGL(fill_jobs_kernel, dim3(1), dim3(256), 0, s, dst, c, n);
+}
 
 std::vector<KernelEntry>& registry() {
   static std::vector<KernelEntry> r;
@@ -607,6 +638,7 @@ constexpr size_t kMaxGraphs = 512;  // cached graph executables per context
 struct sth_ctx {
   int device = 0;
   int cus = 0;
+  sth::BatchJob* scratch = nullptr;  // this device's copy of `scratch` (sth_map_batch_static)
   hipStream_t capture = nullptr;  // private stream used only for graph capture
   std::mutex mu;
   std::map<std::string, hipGraphExec_t> graphs;
@@ -715,13 +747,15 @@ int tune_pick(sth_ctx* ctx, const Plan& pl, const std::vector<Variant>& c, const
   return STH_OK;
 }
 
-// One step of many independent calls: calls with a batched variant go into
-// one launch per (element type, shape, reduction) after their halo refresh;
-// the rest, and calls that fill the device alone, run as single launches
-// first.  `serial_only` runs every call alone.
-int launch_batched(sth_ctx* ctx, const Plan* const* plans, int32_t n, bool serial_only, hipStream_t s) {
+// One step of many independent calls, first half: calls without a batched
+// variant, and calls that fill the device alone, run here as single launches
+// on `s`; the rest are grouped per batched kernel ((element type, variant)
+// key), each after its halo refresh.  `serial_only` runs every call alone.
+using BatchGroups = std::map<int64_t, std::vector<BatchJob>>;
+
+int group_batched(sth_ctx* ctx, const Plan* const* plans, int32_t n, bool serial_only, hipStream_t s,
+                  BatchGroups* groups) {
   int rc = STH_OK;
-  std::map<int64_t, std::vector<BatchJob>> groups;
   for (int32_t i = 0; i < n; ++i) {
     const Plan& pl = *plans[i];
     if (serial_only || pl.bvariant < 0 || !pl.fn || runs_alone(pl, ctx->cus)) {
@@ -747,28 +781,54 @@ int launch_batched(sth_ctx* ctx, const Plan* const* plans, int32_t n, bool seria
     bj.tiles_x = static_cast<int32_t>((pl.p.src.n0 + lpr * pl.bsv - 1) / (lpr * pl.bsv));
     bj.tiles_y = static_cast<int32_t>((pl.p.src.n1 + 4 * pl.bsky * rpw - 1) / (4 * pl.bsky * rpw));
     bj.tiles_z = static_cast<int32_t>(pl.p.src.n2);
-    groups[static_cast<int64_t>(pl.dtype) * 65536 + pl.bklass].push_back(bj);
+    (*groups)[static_cast<int64_t>(pl.dtype) * 65536 + pl.bklass].push_back(bj);
   }
+  return STH_OK;
+}
+
+// Second half, per group: its kernel, the jobs' first tiles and the launch
+// shape (one tile per workgroup: a workgroup's tiles run one after another
+// with no overlap inside a wave; STH_BATCH_TPB sets strips).  Sets *total to
+// 0 for a group with no tiles.
+int prepare_group(int64_t key, std::vector<BatchJob>& bj, const BatchKernelEntry** ke, BatchParams* bp,
+                  int64_t* blocks) {
+  *ke = nullptr;
+  for (const auto& e : batch_kernels())
+    if (static_cast<int64_t>(e.dtype) * 65536 + e.klass == key) *ke = &e;
+  if (!*ke) return fail(STH_EDEVICE, "no batched kernel for dtype %d variant %d", static_cast<int>(key / 65536),
+                        static_cast<int>(key % 65536));
+  int64_t total = 0;
+  for (auto& j : bj) {
+    j.tile_begin = total;
+    total += static_cast<int64_t>(j.tiles_x) * j.tiles_y * j.tiles_z;
+  }
+  int64_t tpb = 1;
+  if (const char* e = std::getenv("STH_BATCH_TPB")) tpb = std::max(1, std::atoi(e));
+  *blocks = (total + tpb - 1) / tpb;
+  if (*blocks > (int64_t(1) << 31) - 1)
+    return fail(STH_ENOTSUP, "batch of %lld tiles is too large", static_cast<long long>(total));
+  bp->jobs = nullptr;
+  bp->njobs = static_cast<int32_t>(bj.size());
+  bp->tiles_per_block = static_cast<int32_t>(tpb);
+  bp->total_tiles = total;
+  return STH_OK;
+}
+
+// One step of many independent calls: calls with a batched variant go into
+// one launch per (element type, shape, reduction) after their halo refresh;
+// the rest, and calls that fill the device alone, run as single launches
+// first.  The job tables are uploaded into device slots the context owns.
+int launch_batched(sth_ctx* ctx, const Plan* const* plans, int32_t n, bool serial_only, hipStream_t s) {
+  BatchGroups groups;
+  int rc = group_batched(ctx, plans, n, serial_only, s, &groups);
+  if (rc) return rc;
   for (auto& kv : groups) {
-    const BatchKernelEntry* ke = nullptr;
-    for (const auto& e : batch_kernels())
-      if (static_cast<int64_t>(e.dtype) * 65536 + e.klass == kv.first) ke = &e;
-    if (!ke) return fail(STH_EDEVICE, "no batched kernel for dtype %d variant %d", static_cast<int>(kv.first / 65536),
-                           static_cast<int>(kv.first % 65536));
     std::vector<BatchJob>& bj = kv.second;
-    int64_t total = 0;
-    for (auto& j : bj) {
-      j.tile_begin = total;
-      total += static_cast<int64_t>(j.tiles_x) * j.tiles_y * j.tiles_z;
-    }
-    if (total == 0) continue;
-    // one tile per workgroup (measured fastest: a workgroup's tiles run one
-    // after another with no overlap inside a wave); STH_BATCH_TPB sets strips
-    int64_t tpb = 1;
-    if (const char* e = std::getenv("STH_BATCH_TPB")) tpb = std::max(1, std::atoi(e));
-    const int64_t blocks = (total + tpb - 1) / tpb;
-    if (blocks > (int64_t(1) << 31) - 1)
-      return fail(STH_ENOTSUP, "batch of %lld tiles is too large", static_cast<long long>(total));
+    const BatchKernelEntry* ke = nullptr;
+    BatchParams bp;
+    int64_t blocks = 0;
+    if ((rc = prepare_group(kv.first, bj, &ke, &bp, &blocks))) return rc;
+    if (bp.total_tiles == 0) continue;
     const size_t bytes = bj.size() * sizeof(BatchJob);
     std::lock_guard<std::mutex> lk(ctx->mu);
     BatchSlot* hit = nullptr;
@@ -803,15 +863,49 @@ int launch_batched(sth_ctx* ctx, const Plan* const* plans, int32_t n, bool seria
       // they ran on (so an overwrite that waits for `done` waits for all)
       STH_HIP(hipStreamWaitEvent(s, sl.done, 0));
     }
-    BatchParams bp;
     bp.jobs = static_cast<const BatchJob*>(sl.dev);
-    bp.njobs = static_cast<int32_t>(bj.size());
-    bp.tiles_per_block = static_cast<int32_t>(tpb);
-    bp.total_tiles = total;
     ke->fn(bp, static_cast<unsigned>(blocks), s);
     STH_HIP(hipGetLastError());
     STH_HIP(hipEventRecord(sl.done, s));
     sl.used = true;
+  }
+  return STH_OK;
+}
+
+// The same step with every job table in `table` (the device's fixed
+// `scratch` array, at most kStaticBatchJobs entries): each group's table is
+// written by fill_jobs_kernel launches whose arguments carry the jobs, so the
+// call copies nothing from host memory and allocates nothing.
+int launch_batched_static(sth_ctx* ctx, const Plan* const* plans, int32_t n, bool serial_only, hipStream_t s,
+                          BatchJob* table) {
+  BatchGroups groups;
+  int rc = group_batched(ctx, plans, n, serial_only, s, &groups);
+  if (rc) return rc;
+  int64_t used = 0;
+  for (auto& kv : groups) used += static_cast<int64_t>(kv.second.size());
+  if (used > kStaticBatchJobs)
+    return fail(STH_EINVAL, "%lld batched jobs exceed the fixed table of %d", static_cast<long long>(used),
+                kStaticBatchJobs);
+  int64_t off = 0;
+  for (auto& kv : groups) {
+    std::vector<BatchJob>& bj = kv.second;
+    const BatchKernelEntry* ke = nullptr;
+    BatchParams bp;
+    int64_t blocks = 0;
+    if ((rc = prepare_group(kv.first, bj, &ke, &bp, &blocks))) return rc;
+    if (bp.total_tiles == 0) continue;
+    for (size_t i = 0; i < bj.size(); i += kFillJobs) {
+      JobChunk c;
+      std::memset(static_cast<void*>(&c), 0, sizeof c);
+      const int m = static_cast<int>(std::min<size_t>(kFillJobs, bj.size() - i));
+      std::memcpy(static_cast<void*>(c.j), bj.data() + i, m * sizeof(BatchJob));
+      launch_fill_jobs(table + off + static_cast<int64_t>(i), c, m, s);
+      STH_HIP(hipGetLastError());
+    }
+    bp.jobs = table + off;
+    ke->fn(bp, static_cast<unsigned>(blocks), s);
+    STH_HIP(hipGetLastError());
+    off += static_cast<int64_t>(bj.size());
   }
   return STH_OK;
 }
@@ -854,6 +948,14 @@ int sth_ctx_create(int device, sth_ctx** out) {
   if (e != hipSuccess) {
     delete c;
     return fail(STH_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+  }
+  // the device address of this device's `scratch` array (the name itself is
+  // a host-side stand-in that kernels must never receive)
+  e = hipGetSymbolAddress(reinterpret_cast<void**>(&c->scratch), HIP_SYMBOL(scratch));
+  if (e != hipSuccess) {
+    (void)hipStreamDestroy(c->capture);
+    delete c;
+    return fail(STH_EDEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
   }
   if (const char* v = std::getenv("STH_AUTOTUNE")) c->autotune = std::strcmp(v, "0") != 0;
   if (const char* v = std::getenv("STH_TUNE_FILE")) c->tune_file = v;
@@ -1181,6 +1283,39 @@ int sth_map_batch(sth_ctx* ctx, int32_t njobs, const sth_job* jobs, uint32_t fla
     ptrs[i] = &plans[i];
   }
   if ((rc = launch_batched(ctx, ptrs.data(), njobs, serial_only, s))) return rc;
+  if (!(flags & STH_ASYNC)) STH_HIP(hipStreamSynchronize(s));
+  return STH_OK;
+  STH_GUARD_END
+}
+
+int sth_map_batch_static(sth_ctx* ctx, int32_t njobs, const sth_job* jobs, uint32_t flags, void* stream) {
+  STH_GUARD_BEGIN
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  if (njobs < 0 || (njobs > 0 && !jobs)) return fail(STH_EINVAL, "bad job list");
+  if (njobs > kStaticBatchJobs)
+    return fail(STH_EINVAL, "%d jobs exceed STH_STATIC_BATCH_MAX_JOBS (%d)", njobs, kStaticBatchJobs);
+  if (!ctx->scratch) return fail(STH_EDEVICE, "the fixed job table of this device is not mapped");
+  auto s = static_cast<hipStream_t>(stream);
+  const bool serial_only = (flags & STH_BATCH_SERIAL) != 0;
+  std::vector<Plan> plans(static_cast<size_t>(njobs));
+  std::vector<const Plan*> ptrs(static_cast<size_t>(njobs));
+  for (int32_t i = 0; i < njobs; ++i) {
+    const sth_job& jb = jobs[i];
+    std::vector<Variant> cands;
+    std::string group;
+    if ((rc = make_plan(&jb.src, &jb.dst, &jb.stencil, jb.boundary, jb.padval, jb.padding, &jb.op,
+                        flags & ~STH_GENERIC, &plans[i], &cands, &group))) {
+      std::string m = g_err;
+      return fail(rc, "job %d: %s", i, m.c_str());
+    }
+    // decisions the tuner already made only: timing would create events
+    int idx = 0;
+    if ((rc = tune_pick(ctx, plans[i], cands, group, flags, s, false, true, &idx))) return rc;
+    apply_variant(plans[i], cands[idx]);
+    ptrs[i] = &plans[i];
+  }
+  if ((rc = launch_batched_static(ctx, ptrs.data(), njobs, serial_only, s, ctx->scratch))) return rc;
   if (!(flags & STH_ASYNC)) STH_HIP(hipStreamSynchronize(s));
   return STH_OK;
   STH_GUARD_END

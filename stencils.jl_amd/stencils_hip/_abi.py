@@ -44,6 +44,7 @@ ASYNC = 1
 GRAPH = 2
 GENERIC = 4
 BATCH_SERIAL = 8
+STATIC_BATCH_MAX_JOBS = 4096  # sth_map_batch_static's fixed job table
 NO_TUNE = 16
 NO_FUSE = 32
 

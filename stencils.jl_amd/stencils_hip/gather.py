@@ -344,6 +344,18 @@ class BatchPlan:
         _lib.check(L.sth_map_batch(_lib.context(self.dev), self.n, self.jobs, fl,
                                    _stream(self.dev)))
 
+    def run_static(self, flags: int = 0):
+        """The same calls through sth_map_batch_static, which allocates no
+        memory (its job tables live in a fixed per-device array of
+        STATIC_BATCH_MAX_JOBS entries shared by every such call)."""
+        if self.n == 0:
+            return
+        if self.n > _abi.STATIC_BATCH_MAX_JOBS:
+            raise ValueError(f"{self.n} jobs exceed the fixed table of {_abi.STATIC_BATCH_MAX_JOBS}")
+        L = _lib.lib()
+        _lib.check(L.sth_map_batch_static(_lib.context(self.dev), self.n, self.jobs, flags,
+                                          _stream(self.dev)))
+
 
 def mapstencil_batch(calls, serial: bool = False, flags: int = 0):
     """Many gatherstencil!(f, dest, source) calls in one library call
