@@ -67,22 +67,29 @@ def kernel_launch_us(sh, torch, S, rule, launches: int = 200):
     return S, e0.elapsed_time(e1) / launches * 1e3
 
 
-def life2_launch_us(sh, torch, S, rule, m1: int = 2, m2: int = 17):
-    """Average duration of one life2_kernel launch (two generations).  A call
-    of 3 + 64 m generations runs one eager step, m replays of the captured
-    16-unit graph (32 life2_kernel launches each) and the same 2-step tail
-    for every m, so the difference of two such calls, timed between HIP
-    events on the launch stream, is (m2 - m1) x 32 launches of the kernel
-    alone.  Both calls' graphs are captured before timing."""
+# Generations one launch of the steady-state Life kernel advances (the
+# library's default; STH_LIFEB_G overrides it there and here).
+LIFE_GENS = int(os.environ.get("STH_LIFEB_G", "8"))
+
+
+def fused_launch_us(sh, torch, S, rule, gens: int, m1: int = 1, m2: int = 5):
+    """Average duration of one launch of the multi-generation Life kernel
+    (`gens` generations per launch).  A call of 3 + 32 gens m generations
+    runs one eager step, m replays of the captured 16-unit graph (32 fused
+    launches each) and the same 2-step tail for every m, so the difference
+    of two such calls, timed between HIP events on the launch stream, is
+    (m2 - m1) x 32 launches of the kernel alone.  Both calls' graphs are
+    captured before timing."""
     stream = torch.cuda.current_stream()
+    steps = lambda m: 3 + 32 * gens * m  # noqa: E731
     for m in (m1, m2, m1, m2):  # capture both keys at both buffer parities
-        S = sh.mapstencil_steps(rule, S, 3 + 64 * m, graph=True, flags=sh._abi.ASYNC)
+        S = sh.mapstencil_steps(rule, S, steps(m), graph=True, flags=sh._abi.ASYNC)
     t = {}
     for m in (m1, m2):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
         e0.record(stream)
-        S = sh.mapstencil_steps(rule, S, 3 + 64 * m, graph=True, flags=sh._abi.ASYNC)
+        S = sh.mapstencil_steps(rule, S, steps(m), graph=True, flags=sh._abi.ASYNC)
         e1.record(stream)
         torch.cuda.synchronize()
         t[m] = e0.elapsed_time(e1) * 1e3  # us
@@ -431,7 +438,56 @@ def bench_heat_iterate(sh, torch, n: int = 512, steps: int = 200):
                      "one_step_achieved_gbs": 2 * es * cells / (t["one_step"] * 1e-6) / 1e9}
         if name == "f32" and n == 512:
             res[name]["two_step_roofline"]["traffic"] = profiled_traffic("zmarch2_kernel_heat_512.md")
+            res[name]["one_step_traffic"] = profiled_traffic("zmarch1_kernel_heat_512.md")
         del A, S
+        torch.cuda.empty_cache()
+    return res
+
+
+def bench_life_only(sh, torch, steps: int, n: int = 8192):
+    """The headline Game of Life leg alone (for profiling): `steps`
+    graph-replayed generations after an untimed pass that captures the
+    graphs, then one 256 MiB device-to-device copy whose HBM counters
+    calibrate FETCH_SIZE / WRITE_SIZE."""
+    A = life_grid(n)
+    rule = sh.LifeRule()
+    S = device_switching(A, sh, torch)
+    S, _, _ = time_steps(sh, torch, S, rule, steps, graph=True)
+    S, wall, ev = time_steps(sh, torch, S, rule, steps, graph=True)
+    src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda").fill_(1)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    return {"workload": f"game-of-life {n}x{n}, {steps} generations, hipGraph", "gens_per_launch": LIFE_GENS,
+            "us_per_generation": ev / steps * 1e6, "cells_per_s": n * n * steps / wall,
+            "calibration_copy_bytes": 256 << 20}
+
+
+def bench_heat3d_step(sh, torch, n: int = 512, reps: int = 10):
+    """One step of the 3-D heat stencil (VonNeumann(1,3) Laplacian(0.15),
+    Wrap) on n^3, Float32, BF16 and Float64, as `reps` single
+    gatherstencil! calls with the tuned kernel (for profiling the one-step
+    body)."""
+    f = sh.Laplacian(0.15)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    res = {"workload": f"heat {n}^3 one step, VonNeumann(1,3) Laplacian(0.15) Wrap"}
+    for name, tdt in (("f32", torch.float32), ("bf16", torch.bfloat16), ("f64", torch.float64)):
+        A = sh.colmajor_empty((n,) * 3, tdt, dev)
+        A.copy_(torch.rand((n,) * 3, device=dev).to(tdt))
+        sa = sh.StencilArray(A, sh.VonNeumann(1, 3), boundary=sh.Wrap())
+        dest = sh.colmajor_empty((n,) * 3, tdt, dev)
+        kernel = sh.autotune(f, dest, sa)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            sh.mapstencil_(f, dest, sa, flags=sh._abi.ASYNC)
+        e1.record()
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) / reps * 1e3
+        es = A.element_size()
+        res[name] = {"kernel": kernel, "us": us, "achieved_gbs": 2 * es * n ** 3 / (us * 1e-6) / 1e9}
+        del A, dest, sa
         torch.cuda.empty_cache()
     return res
 
@@ -482,7 +538,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the secondary workloads (mean blur, Gaussian)")
-    ap.add_argument("--only", choices=["heat3d", "heat2d", "served", "batched", "gaussian", "meanblur"],
+    ap.add_argument("--only", choices=["life", "heat3d-step", "heat3d", "heat2d", "served", "batched", "gaussian",
+                                       "meanblur"],
                     help="run one secondary workload alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -496,7 +553,9 @@ def main():
     sh.load_library()
     torch.cuda.set_device(int(os.environ.get("LOCAL_RANK", 0)))
     if args.only:
-        run_one = {"heat3d": lambda: bench_heat_iterate(sh, torch, steps=args.steps),
+        run_one = {"life": lambda: bench_life_only(sh, torch, args.steps),
+                   "heat3d-step": lambda: bench_heat3d_step(sh, torch),
+                   "heat3d": lambda: bench_heat_iterate(sh, torch, steps=args.steps),
                    "heat2d": lambda: bench_heat_iterate_2d(sh, torch),
                    "served": lambda: bench_served(sh, torch),
                    "batched": lambda: bench_batched(sh, torch),
@@ -520,11 +579,11 @@ def main():
     S, wall, ev = time_steps(sh, torch, S, rule, args.steps, graph=True)
     bytes_per_step = 2 * cells  # 1 B read + 1 B written per interior cell and generation
     gen_us = ev / args.steps * 1e6
-    # The graph-replayed steady steps run life2_kernel: two generations per
-    # launch (temporal blocking), so a launch's algorithmic bytes cover two
-    # generations.  Its duration is timed on its own (life2_launch_us).
-    S, launch_us = life2_launch_us(sh, torch, S, rule)
-    bytes_per_launch = 2 * bytes_per_step
+    # The graph-replayed steady steps run lifeb_kernel: LIFE_GENS generations
+    # per launch (temporal blocking), so a launch's algorithmic bytes cover
+    # LIFE_GENS generations.  Its duration is timed on its own.
+    S, launch_us = fused_launch_us(sh, torch, S, rule, LIFE_GENS)
+    bytes_per_launch = LIFE_GENS * bytes_per_step
     achieved = bytes_per_launch / (launch_us * 1e-6) / 1e9
     S, plain_launch_us = kernel_launch_us(sh, torch, S, rule)
     # measured stream-copy bandwidth on the same bytes (one read + one write)
@@ -563,8 +622,9 @@ def main():
                                "steps", "cells": cells, "step": "one generation"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic("life2_kernel_gol_8192.md"),
-                     "kernel": "life2_kernel<KY=8, B3/S23>: two generations per launch",
+                     "traffic": profiled_traffic("lifeb_kernel_gol_8192.md"),
+                     "kernel": f"lifeb_kernel<G={LIFE_GENS}>: {LIFE_GENS} generations per launch, one bit "
+                               "per cell",
                      "bytes_per_launch": bytes_per_launch, "avg_launch_us": launch_us,
                      "per_generation_us": gen_us,
                      "note": "algorithmic bytes count 2 B per cell and generation (temporal-blocking "

@@ -98,7 +98,7 @@ typedef enum {
 /* The fixed reduction set accelerated on the GPU. */
 typedef enum {
   STH_OP_SUM = 0,           /* StaticArrays sum: left fold in offset order   */
-  STH_OP_MEAN = 1,          /* Statistics mean: fold in float(T), / L        */
+  STH_OP_MEAN = 1,          /* sum / L (integers: see STH_OPF_INT_WRAP)      */
   STH_OP_MIN = 2,           /* minimum: left fold with Julia `min`           */
   STH_OP_MAX = 3,           /* maximum: left fold with Julia `max`           */
   STH_OP_KERNELPRODUCT = 4, /* kernelproduct, src/stencils/kernel.jl:37-43   */

@@ -17,8 +17,10 @@
 #include "batch_kernel.hpp"
 #include "halo_kernel.hpp"
 #include "zmarch2_kernel.hpp"
+#include "zmarch1_kernel.hpp"
 #include "ymarch2_kernel.hpp"
 #include "life2_kernel.hpp"
+#include "lifeb_kernel.hpp"
 #include "life_kernel.hpp"
 #include "registry.hpp"
 #include "stencils_hip.h"
@@ -134,7 +136,8 @@ struct Plan {
   int32_t bsv, bsky; // its streaming tile (64*bsv cells x 4*bsky rows)
   int32_t bklass;    // which batched kernel (one per (shape, op) variant)
   int32_t dtype;     // sth_dtype of the source
-  LaunchFn fn_fused2;     // two steps per launch (steady steps), or null
+  LaunchFn fn_fused;      // several steps per launch (steady steps), or null
+  int32_t fused_gens;      // steps fn_fused advances per launch
   int32_t fused_tiles_x, fused_tiles_y, fused_tiles_z;  // its grid
   HaloParams hp;
   void (*halo_fn)(const HaloParams&, hipStream_t);
@@ -399,14 +402,39 @@ int make_plan(const sth_array* src, const sth_array* dst, const sth_stencil* st,
       p.read_halo = 0;
       p.fuse_halo = 1;
     }
-    if (bc != STH_USE && n[1] >= 3) {
+    // Conway's rule under Wrap / Remove / Reflect on grids below 2 GiB:
+    // several generations per launch with one bit per cell (lifeb_kernel)
+    const bool bitlife = conway && (bc == STH_WRAP || bc == STH_REMOVE || bc == STH_REFLECT) && n[0] >= 16 &&
+                         zm2_extent(p.src, 1) < kZm2MaxBytes && zm2_extent(p.dst, 1) < kZm2MaxBytes;
+    if (bc != STH_USE && n[1] >= 3 && bitlife) {
+      // generations per launch (STH_LIFEB_G: 2, 4 or 8) and waves per launch
+      // (STH_LIFEB_WAVES), read per call
+      // (8 measured fastest on 8192^2: 3.6 us per generation, against 5.8
+      // with 4 and 10.7 with 2)
+      const char* eg = std::getenv("STH_LIFEB_G");
+      const int lg0 = eg ? std::atoi(eg) : 8;
+      const int lg = lg0 == 2 || lg0 == 4 ? lg0 : 8;
+      const char* ew = std::getenv("STH_LIFEB_WAVES");
+      const int64_t lwaves = ew ? std::max<int64_t>(1, std::atoll(ew)) : int64_t(2048);
+      // plain stores: the next launch reads them back, from the Infinity
+      // Cache when the grid fits (STH_LIFEB_NT=1: non-temporal, to compare)
+      const char* en = std::getenv("STH_LIFEB_NT");
+      const bool lnt = en && std::strcmp(en, "1") == 0;
+      pl.fn_fused = lg == 2 ? &launch_lifeb<2, 6, 0>
+                  : lg == 4 ? &launch_lifeb<4, 6, 0>
+                            : (lnt ? &launch_lifeb<8, 6, 1> : &launch_lifeb<8, 6, 0>);
+      pl.fused_gens = lg;
+      lifeb_grid(n[0], n[1], lg, lwaves, &pl.fused_tiles_x, &pl.fused_tiles_y);
+      pl.fused_tiles_z = 1;
+    } else if (bc != STH_USE && n[1] >= 3) {
       // temporal blocking for steady iteration steps (two generations per launch)
       int ky2 = 8;  // rows per wave of the fused kernel (measured best; STH_LIFE2_KY: 2, 4 or 8)
       if (const char* e = std::getenv("STH_LIFE2_KY")) ky2 = std::atoi(e);
       if (ky2 != 2 && ky2 != 4) ky2 = 8;
-      pl.fn_fused2 = ky2 == 2 ? (conway ? &launch_life2<2, true> : &launch_life2<2, false>)
+      pl.fn_fused = ky2 == 2 ? (conway ? &launch_life2<2, true> : &launch_life2<2, false>)
                    : ky2 == 8 ? (conway ? &launch_life2<8, true> : &launch_life2<8, false>)
                               : (conway ? &launch_life2<4, true> : &launch_life2<4, false>);
+      pl.fused_gens = 2;
       pl.fused_tiles_x = static_cast<int32_t>((n[0] + 1023) / 1024);
       pl.fused_tiles_y = static_cast<int32_t>((n[1] + 4 * ky2 - 1) / (4 * ky2));
       pl.fused_tiles_z = 1;
@@ -431,6 +459,25 @@ int make_plan(const sth_array* src, const sth_array* dst, const sth_stencil* st,
           v.tiles_y = static_cast<int32_t>((n[1] + 4 * e.sky * rpw - 1) / (4 * e.sky * rpw));
           if (n[2] > (int64_t(1) << 31) / 2) continue;
           v.tiles_z = static_cast<int32_t>(n[2]);
+        } else if (e.cfg == CFG_ZMARCH1) {
+          // z-marching one-step kernel: 3-D arrays under Wrap / Remove /
+          // Reflect, rows a whole number of V-cell vectors on V-cell
+          // boundaries, byte extents within its 32-bit buffer offsets
+          const int es = dtype_size(sdt), eo = dtype_size(odt);
+          const auto vec_aligned = [&](const DevArr& a, int esz) {
+            const int64_t vb = static_cast<int64_t>(e.sv) * esz;
+            return reinterpret_cast<uintptr_t>(a.base) % vb == 0 && (a.s1 * esz) % vb == 0 &&
+                   (a.s2 * esz) % vb == 0;
+          };
+          if (src->ndim != 3 || !(bc == STH_WRAP || bc == STH_REMOVE || bc == STH_REFLECT) ||
+              n[0] % e.sv != 0 || !vec_aligned(p.src, es) || !vec_aligned(p.dst, eo) ||
+              zm2_extent(p.src, es) >= kZm2MaxBytes || zm2_extent(p.dst, eo) >= kZm2MaxBytes)
+            continue;
+          static const int64_t zm1_waves = [] {
+            const char* w = std::getenv("STH_ZM1_WAVES");
+            return w ? std::max<int64_t>(1, std::atoll(w)) : int64_t(8192);
+          }();
+          zmarch1_grid(n[0], n[1], n[2], e.sv, e.sky, zm1_waves, &v.tiles_x, &v.tiles_y, &v.tiles_z);
         } else {
           const TileDims td = tile_dims(e.cfg);
           const int staged = (sdt == STH_BF16 || sdt == STH_F16) ? 4 : dtype_size(sdt);
@@ -492,7 +539,7 @@ int make_plan(const sth_array* src, const sth_array* dst, const sth_stencil* st,
     const char* e = std::getenv("STH_FUSE3D_MIN_CELLS");
     return e ? std::atoll(e) : (int64_t(1) << 21);
   }();
-  if (!pl.fn_fused2 && !(flags & (STH_NO_FUSE | STH_GENERIC)) && src->ndim == 3 && st->ndim == 3 &&
+  if (!pl.fn_fused && !(flags & (STH_NO_FUSE | STH_GENERIC)) && src->ndim == 3 && st->ndim == 3 &&
       (bc == STH_WRAP || bc == STH_REMOVE || bc == STH_REFLECT) && n[0] >= 3 && n[1] >= 3 && n[2] >= 3 &&
       n[0] * n[1] * n[2] >= fuse3d_min &&
       zm2_extent(pl.p.src, dtype_size(sdt)) < kZm2MaxBytes && zm2_extent(pl.p.dst, dtype_size(odt)) < kZm2MaxBytes) {
@@ -503,7 +550,8 @@ int make_plan(const sth_array* src, const sth_array* dst, const sth_stencil* st,
         const char* v = std::getenv("STH_ZM2_WAVES");
         return v ? std::max<int64_t>(1, std::atoll(v)) : int64_t(8192);
       }();
-      pl.fn_fused2 = e.fn;
+      pl.fn_fused = e.fn;
+      pl.fused_gens = 2;
       zmarch2_grid(n[0], n[1], n[2], e.rows, zm_waves, &pl.fused_tiles_x, &pl.fused_tiles_y, &pl.fused_tiles_z);
       break;
     }
@@ -521,7 +569,7 @@ int make_plan(const sth_array* src, const sth_array* dst, const sth_stencil* st,
   auto rows16 = [](const DevArr& a, int64_t es) {
     return reinterpret_cast<uintptr_t>(a.base) % 16 == 0 && (a.s1 * es) % 16 == 0;
   };
-  if (!pl.fn_fused2 && !(flags & (STH_NO_FUSE | STH_GENERIC)) && src->ndim == 2 && st->ndim == 2 &&
+  if (!pl.fn_fused && !(flags & (STH_NO_FUSE | STH_GENERIC)) && src->ndim == 2 && st->ndim == 2 &&
       (bc == STH_WRAP || bc == STH_REMOVE || bc == STH_REFLECT) && n[0] >= 3 && n[1] >= 3 &&
       n[0] * n[1] >= fuse2d_min &&
       sdt == odt && rows16(pl.p.src, dtype_size(sdt)) && rows16(pl.p.dst, dtype_size(odt)) &&
@@ -533,7 +581,8 @@ int make_plan(const sth_array* src, const sth_array* dst, const sth_stencil* st,
         const char* v = std::getenv("STH_YM2_WAVES");
         return v ? std::max<int64_t>(1, std::atoll(v)) : int64_t(8192);
       }();
-      pl.fn_fused2 = e.fn;
+      pl.fn_fused = e.fn;
+      pl.fused_gens = 2;
       ymarch2_grid(n[0], n[1], e.rows, ym_waves, &pl.fused_tiles_x, &pl.fused_tiles_y, &pl.fused_tiles_z);
       break;
     }
@@ -590,16 +639,16 @@ int run_plan(const Plan& pl, hipStream_t s, bool steady = false) {
   return STH_OK;
 }
 
-// Two steps in one launch (pl.fn_fused2, steady state).  The fused kernels
+// Several steps in one launch (pl.fn_fused, steady state).  The fused kernels
 // map every boundary themselves and write only the destination's interior.
-int run_fused2(const Plan& pl, hipStream_t s) {
+int run_fused(const Plan& pl, hipStream_t s) {
   MapParams q = pl.p;
   q.tiles_x = pl.fused_tiles_x;
   q.tiles_y = pl.fused_tiles_y;
   q.tiles_z = pl.fused_tiles_z;
   q.read_halo = 0;
   q.fuse_halo = 0;
-  pl.fn_fused2(q, s);
+  pl.fn_fused(q, s);
   STH_HIP(hipGetLastError());
   return STH_OK;
 }
@@ -1169,19 +1218,22 @@ int sth_iterate(sth_ctx* ctx, const sth_array* a, const sth_array* b, const sth_
   // write); steps 1.. use the steady-state kernels.
   if ((rc = run_plan(ab, s))) return rc;
   const int64_t rest = steps - 1;
-  // Temporal blocking: two steps per launch.  Buffers then hold generations
-  // one apart from the step-by-step order, so fused launches come in pairs
-  // (a unit: b -> a, a -> b, four steps, buffers back in step), and every
-  // iteration ends with at least two single steps, which leave both buffers
-  // and their halos as the step-by-step reference does.
-  const bool fuse = !(flags & STH_NO_FUSE) && ab.fn_fused2 && ba.fn_fused2 && rest >= 6;
-  const int64_t units = fuse ? (rest - 2) / 4 : 0;  // fused pairs
-  const int64_t tail = rest - 4 * units;            // single steps (2 .. 5 when fused)
+  // Temporal blocking: G = fused_gens steps per launch.  A fused launch
+  // writes only the newest generation, so the buffer it reads is left G - 1
+  // generations behind the step-by-step order; fused launches come in pairs
+  // (a unit: b -> a, a -> b, 2 G steps, the write buffer alternating as in
+  // the step-by-step order), and every iteration ends with at least two
+  // single steps, which leave both buffers and their halos as the
+  // step-by-step reference does.
+  const int64_t G = ab.fused_gens;
+  const bool fuse = !(flags & STH_NO_FUSE) && ab.fn_fused && ba.fn_fused && G >= 2 && rest >= 2 * G + 2;
+  const int64_t units = fuse ? (rest - 2) / (2 * G) : 0;  // fused pairs
+  const int64_t tail = rest - 2 * G * units;              // single steps (2 .. 2G + 1 when fused)
   auto enqueue = [&](hipStream_t st, int64_t nunits, int64_t nplain) -> int {
     int r2 = STH_OK;
     for (int64_t u = 0; u < nunits && r2 == STH_OK; ++u) {
-      r2 = run_fused2(ba, st);
-      if (r2 == STH_OK) r2 = run_fused2(ab, st);
+      r2 = run_fused(ba, st);
+      if (r2 == STH_OK) r2 = run_fused(ab, st);
     }
     for (int64_t i = 0; i < nplain && r2 == STH_OK; ++i) r2 = run_plan(i % 2 ? ab : ba, st, true);
     return r2;

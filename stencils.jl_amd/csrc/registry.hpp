@@ -19,7 +19,8 @@ enum TileCfg : int {
   CFG_RT2D = 3,      // 64 x 16 x 1, runtime offsets
   CFG_RT3D = 4,      // 32 x 8 x 4, runtime offsets
   CFG_LIFE = 5,      // SWAR Life kernel (1024 x 4*KY cells per block)
-  CFG_STREAM = 6     // streaming kernel: 64*V x 4*KY cells of one plane per block
+  CFG_STREAM = 6,    // streaming kernel: 64*V x 4*KY cells of one plane per block
+  CFG_ZMARCH1 = 7    // z-marching one-step kernel (3-D, reach 1): 64*V x 4*R cells x a z-chunk
 };
 
 struct TileDims {
@@ -45,8 +46,8 @@ struct KernelEntry {
   const int8_t* offs;    // static shapes: L x 3 offsets
   LaunchFn fn;
   const char* name;
-  int sv = 0;            // streaming kernels: cells per lane along dim 1
-  int sky = 0;           // streaming kernels: rows per wave along dim 2
+  int sv = 0;            // streaming / z-marching kernels: cells per lane along dim 1
+  int sky = 0;           // streaming kernels: rows per lane; z-marching: rows per wave
 };
 
 std::vector<KernelEntry>& registry();

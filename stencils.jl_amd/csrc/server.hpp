@@ -276,7 +276,7 @@ struct sth_server {
     return v;
   }
   bool solo(const Req& q) const {
-    return q.plan[0].fn_fused2 && q.plan[1].fn_fused2 && q.cells >= solo_cells();
+    return q.plan[0].fn_fused && q.plan[1].fn_fused && q.cells >= solo_cells();
   }
 
   // `n` steps of a solo request.  Fused launches leave the buffers holding
@@ -296,14 +296,15 @@ struct sth_server {
       n -= 1;
     }
     const int64_t rem = q.r.steps - q.done;
-    const int64_t fused = n == rem ? std::max<int64_t>(0, (n - 2) / 2) : n / 2;
+    const int64_t G = std::max<int32_t>(1, q.plan[0].fused_gens);  // steps per fused launch
+    const int64_t fused = n == rem ? std::max<int64_t>(0, (n - 2) / G) : n / G;
     for (int64_t i = 0; i < fused; ++i) {
-      if (int rc = sth::run_fused2(q.plan[q.cur], st)) return rc;
+      if (int rc = sth::run_fused(q.plan[q.cur], st)) return rc;
       q.cur ^= 1;
-      q.done += 2;
+      q.done += G;
       *launches += 1;
     }
-    for (int64_t i = 2 * fused; i < n; ++i) {
+    for (int64_t i = G * fused; i < n; ++i) {
       if (int rc = sth::run_plan(q.plan[q.cur], st)) return rc;
       q.cur ^= 1;
       q.done += 1;

@@ -286,10 +286,10 @@ __global__ __launch_bounds__(256) void ymarch2_kernel(MapParams p) {
           }
           const uint32_t yoff = static_cast<uint32_t>(y * p.dst.s1 * es);
           if (store_vec) {
-            buf_store<RawVec<ORaw, V>>(out, dst, xs[0], yoff);
+            buf_store<RawVec<ORaw, V>, kStoreNT>(out, dst, xs[0], yoff);
           } else {
   #pragma unroll
-            for (int i = 0; i < V; ++i) buf_store<ORaw>(out.e[i], dst, xs[i], yoff);
+            for (int i = 0; i < V; ++i) buf_store<ORaw, kStoreNT>(out.e[i], dst, xs[i], yoff);
           }
         }
       }

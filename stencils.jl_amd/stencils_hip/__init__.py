@@ -9,7 +9,7 @@ from ._lib import BackendUnavailable, StencilsHipError, load_library
 from .array import (StencilArray, SwitchingStencilArray, colmajor_empty, dest,
                     parent, source, switch)
 from .boundary import Conditional, Halo, Reflect, Remove, Use, Wrap, padval
-from .gather import (BatchPlan, gatherstencil, gatherstencil_, mapstencil, mapstencil_,
+from .gather import (BatchPlan, autotune, gatherstencil, gatherstencil_, mapstencil, mapstencil_,
                      mapstencil_batch, mapstencil_steps, result_dtype)
 from .server import Completion, Server, serve_latency_summary
 from .ops import (Laplacian, LifeRule, Unsupported, integer_fold_wraps, kernelproduct,

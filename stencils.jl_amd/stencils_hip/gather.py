@@ -162,6 +162,30 @@ def _map_device(f, dst, dst_halo, sa: AbstractStencilArray, src_buf, flags: int 
                          flags, _stream(dev)))
 
 
+def autotune(f, dest, sa: AbstractStencilArray) -> str:
+    """Time every kernel variant that can run gatherstencil!(f, dest, sa) on
+    device buffers (sth_autotune), keep the fastest for this call's family,
+    run the call with it and return its name.  A variant forced with
+    `_lib.set_variant` is returned (and run) without timing."""
+    red = resolve(f)
+    _require_backend()
+    dt = sa.sth_dtype()
+    call = _Call()
+    st = stencil_descriptor(sa.stencil, dt, call)
+    op = red.op(_op_flags(dt))
+    pv = _padval_bytes(sa)
+    pvbuf = call.hold(ctypes.create_string_buffer(pv, len(pv))) if pv is not None else None
+    size = sa.size()
+    src = _desc(sa.parent(), size, sa.halo())
+    dsc = _desc(dest, size, 0)
+    dev = _device_index(sa.parent())
+    name = ctypes.create_string_buffer(256)
+    _lib.check(_lib.lib().sth_autotune(_lib.context(dev), ctypes.byref(src), ctypes.byref(dsc),
+                                       ctypes.byref(st), sa.boundary.code, pvbuf, sa.padding.code,
+                                       ctypes.byref(op), _stream(dev), name, len(name)))
+    return name.value.decode()
+
+
 def _require_backend():
     torch = _torch()
     if not torch.cuda.is_available():

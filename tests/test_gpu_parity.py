@@ -265,6 +265,44 @@ def test_life_iterate(gpu, oracle, shape, bname, pad):
             np.testing.assert_array_equal(G.dest, O.dest)
 
 
+@pytest.mark.parametrize("bname", ["remove", "remove1", "wrap", "reflect"])
+@pytest.mark.parametrize("gens", ["2", "4", "8", "8nt"])
+def test_life_bit_kernel_generations(gpu, oracle, monkeypatch, gens, bname):
+    """Conway iterations whose steady launches advance 2, 4 or 8 generations
+    with one bit per cell (8nt: with non-temporal stores): both buffers
+    (halos included) against the step-by-step oracle, for a one-lane grid, a
+    strip with a single active lane, three-row grids (rows reflected or
+    wrapped several times over), Bool cells, and chunkings from one chunk
+    per strip to the shortest."""
+    monkeypatch.setenv("STH_LIFEB_NT", "1" if gens.endswith("nt") else "0")
+    gens = int(gens.rstrip("nt"))
+    monkeypatch.setenv("STH_LIFEB_G", str(gens))
+    bc = {"remove": sh.Remove(np.uint8(0)), "remove1": sh.Remove(np.uint8(1)), "wrap": sh.Wrap(),
+          "reflect": sh.Reflect()}[bname]
+    rng = np.random.default_rng(90 + gens)
+    cases = [((16, 40), "cond"), ((1040, 70), "halo_out"), ((2048, 33), "cond"), ((48, 37), "halo_out"),
+             ((32, 3), "cond"), ((64, 4), "halo_out")]
+    for waves in ("2048", "1", "100000"):
+        monkeypatch.setenv("STH_LIFEB_WAVES", waves)
+        for shape, pad in cases:
+            A = (rng.random(shape) < 0.35).astype(np.uint8)
+            for steps in (2 * gens + 3, 4 * gens + 5, 37):
+                G = sh.SwitchingStencilArray(np.asfortranarray(A), sh.Moore(1), boundary=bc, padding=PADS[pad])
+                O = sh.SwitchingStencilArray(np.asfortranarray(A), sh.Moore(1), boundary=bc, padding=PADS[pad])
+                G = sh.mapstencil_steps(sh.LifeRule(), G, steps, graph=steps % 2 == 1)
+                O = oracle.mapstencil_steps(sh.LifeRule(), O, steps)
+                msg = f"G={gens} waves={waves} {shape} {pad} {steps}"
+                np.testing.assert_array_equal(G.source, O.source, err_msg=msg)
+                np.testing.assert_array_equal(G.dest, O.dest, err_msg=msg)
+    B = rng.random((96, 50)) < 0.4
+    bcb = {"remove": sh.Remove(np.bool_(False)), "remove1": sh.Remove(np.bool_(True))}.get(bname, bc)
+    G = sh.SwitchingStencilArray(np.asfortranarray(B), sh.Moore(1), boundary=bcb)
+    O = sh.SwitchingStencilArray(np.asfortranarray(B), sh.Moore(1), boundary=bcb)
+    G = sh.mapstencil_steps(sh.LifeRule(), G, 4 * gens + 3)
+    O = oracle.mapstencil_steps(sh.LifeRule(), O, 4 * gens + 3)
+    np.testing.assert_array_equal(G.source, O.source)
+
+
 def test_life_device_resident_swar_path(gpu, oracle):
     """Halo{:out} device buffers take the SWAR kernel; compare with the oracle."""
     import torch
@@ -322,6 +360,71 @@ def test_heat_3d_iterate(gpu, oracle, dt):
         G = sh.mapstencil_steps(sh.Laplacian(0.15), G, 21, graph=True)
         O = oracle.mapstencil_steps(sh.Laplacian(0.15), O, 21)
         np.testing.assert_array_equal(np.asarray(G), np.asarray(O))
+
+
+ZM1_VARIANTS = {"float32": ["zm1v2r8d4nt", "zm1v1r16d4nt", "zm1v1r16d4c"],
+                "float64": ["zm1v1r4d6nt", "zm1v1r8d4nt", "zm1v1r4d6c"],
+                "bfloat16": ["zm1v2r8d4nt", "zm1v4r8d4nt", "zm1v2r8d4c"],
+                "float16": ["zm1v2r8d4nt", "zm1v4r8d4nt"]}
+
+
+@pytest.mark.parametrize("pad", ["cond", "halo_out"])
+@pytest.mark.parametrize("bname", ["remove", "remove7", "wrap", "reflect"])
+@pytest.mark.parametrize("tname", list(ZM1_VARIANTS))
+def test_zmarch1_variants(gpu, oracle, tname, bname, pad):
+    """Every z-marching one-step variant (forced by name) on 3-D arrays of
+    reach-1 stencils, against the oracle: partial x tiles, partial row
+    strips, short z chunks, two-cell axes, and widths that are not a whole
+    number of a variant's vectors, where it does not apply and the
+    streaming kernel runs."""
+    import torch
+    from stencils_hip import _lib
+    tdt = getattr(torch, tname)
+    storage = tname in ("bfloat16", "float16")
+    code = {"bfloat16": sh._abi.BF16, "float16": sh._abi.F16}.get(tname)
+    dev = torch.device("cuda")
+    cases = ((sh.VonNeumann(1, 3), sh.Laplacian(0.15)), (sh.Moore(1, 3), sh.mean),
+             (sh.VonNeumann(1, 3), sh.maximum))
+    shapes = [(130, 37, 29), (64, 9, 5), (2, 3, 2), (196, 20, 70), (67, 21, 11)]
+    rng = np.random.default_rng(77)
+    try:
+        for pat in ZM1_VARIANTS[tname]:
+            _lib.set_variant(pat)
+            for shape in shapes:
+                A32 = rng.random(shape).astype(np.float32) * 2 - 1
+                for st, f in cases:
+                    if storage:
+                        pvv = 0.75 if bname == "remove7" else 0.0
+                        bc = {"remove": sh.Remove(0.0), "remove7": sh.Remove(pvv), "wrap": sh.Wrap(),
+                              "reflect": sh.Reflect()}[bname]
+                    else:
+                        bc = BOUNDS[bname](np.dtype(tname))
+                    At = torch.from_numpy(np.ascontiguousarray(A32)).to(tdt)
+                    t = sh.colmajor_empty(shape, tdt, dev)
+                    t.copy_(At.to(dev))
+                    sa = sh.StencilArray(t, st, boundary=bc, padding=PADS[pad])
+                    dest = sh.colmajor_empty(shape, tdt, dev)
+                    chosen = sh.autotune(f, dest, sa)
+                    v = int(pat[pat.index("v") + 1:pat.index("r")])
+                    if shape[0] % v == 0:  # rows a whole number of vectors
+                        assert pat in chosen, (pat, chosen, shape)
+                    else:
+                        assert pat not in chosen, (pat, chosen, shape)
+                    if storage:
+                        raw = np.asfortranarray(sa.parent().cpu().view(torch.int16).numpy().view(np.uint16))
+                        pv = torch.tensor([bc.padval], dtype=tdt).view(torch.int16).numpy().tobytes() \
+                            if isinstance(bc, sh.Remove) else None
+                        o = oracle.map_raw(raw, code, shape, sa.halo(), st.offsets(), 3, bc.code, pv,
+                                           PADS[pad].code, f.op(sh._abi.OPF_ACC_F32))
+                        g = dest.cpu().view(torch.int16).numpy().view(np.uint16)
+                        np.testing.assert_array_equal(g, o.view(np.uint16), err_msg=f"{pat} {shape} {st} {f}")
+                    else:
+                        A = At.numpy()
+                        o = oracle.mapstencil(f, sh.StencilArray(np.asfortranarray(A), st, boundary=bc,
+                                                                 padding=PADS[pad]))
+                        np.testing.assert_array_equal(dest.cpu().numpy(), o, err_msg=f"{pat} {shape} {st} {f}")
+    finally:
+        _lib.set_variant("")
 
 
 @pytest.mark.parametrize("pad", ["cond", "halo_out", "halo_in"])
