@@ -683,6 +683,7 @@ struct BatchSlot {
 constexpr int kBatchSlots = 32;
 
 constexpr size_t kMaxGraphs = 512;  // cached graph executables per context
+constexpr int kBatchStreams = 4;    // most streams a batched call spreads its launches over
 
 struct sth_ctx {
   int device = 0;
@@ -691,6 +692,13 @@ struct sth_ctx {
   hipStream_t capture = nullptr;  // private stream used only for graph capture
   std::mutex mu;
   std::map<std::string, hipGraphExec_t> graphs;
+  // streams a batched call spreads its launches over (default priority),
+  // created on first use, with the fork and join events that order them
+  // after and before the caller's stream
+  std::mutex batch_mu;
+  hipStream_t batch_streams[kBatchStreams] = {};
+  hipEvent_t batch_fork = nullptr;
+  hipEvent_t batch_join[kBatchStreams] = {};
   BatchSlot slots[kBatchSlots];
   uint64_t slot_clock = 0;
   // autotuner: shape family + size bucket -> chosen variant name
@@ -716,6 +724,28 @@ namespace {
 int set_device(sth_ctx* ctx) {
   if (!ctx) return fail(STH_EINVAL, "context is NULL");
   STH_HIP(hipSetDevice(ctx->device));
+  return STH_OK;
+}
+
+// Streams a batched call uses (STH_BATCH_STREAMS, 1..kBatchStreams; 1 keeps
+// every launch on the caller's stream).
+int batch_streams_wanted() {
+  static const int v = [] {
+    const char* e = std::getenv("STH_BATCH_STREAMS");
+    const int k = e ? std::atoi(e) : kBatchStreams;
+    return k < 1 ? 1 : (k > kBatchStreams ? kBatchStreams : k);
+  }();
+  return v;
+}
+
+// Creates the first `n` batch streams and the fork / join events (under
+// ctx->batch_mu).
+int ensure_batch_streams(sth_ctx* ctx, int n) {
+  if (!ctx->batch_fork) STH_HIP(hipEventCreateWithFlags(&ctx->batch_fork, hipEventDisableTiming));
+  for (int k = 0; k < n; ++k) {
+    if (!ctx->batch_streams[k]) STH_HIP(hipStreamCreateWithFlags(&ctx->batch_streams[k], hipStreamNonBlocking));
+    if (!ctx->batch_join[k]) STH_HIP(hipEventCreateWithFlags(&ctx->batch_join[k], hipEventDisableTiming));
+  }
   return STH_OK;
 }
 
@@ -802,16 +832,37 @@ int tune_pick(sth_ctx* ctx, const Plan& pl, const std::vector<Variant>& c, const
 // key), each after its halo refresh.  `serial_only` runs every call alone.
 using BatchGroups = std::map<int64_t, std::vector<BatchJob>>;
 
+// Relative cost of one call, for spreading launches over streams: its cells,
+// weighted by the stencil length (the fold's work).
+double plan_cost(const Plan& pl) {
+  return static_cast<double>(pl.p.src.n0 * pl.p.src.n1 * pl.p.src.n2) * (1.0 + pl.p.L / 8.0);
+}
+
+// Whether calls that fill the device alone run as their own launches of the
+// tuned kernel (STH_BATCH_ALONE=0: every catalogue call joins its group).
+bool batch_alone_enabled() {
+  static const bool v = [] {
+    const char* e = std::getenv("STH_BATCH_ALONE");
+    return !(e && std::strcmp(e, "0") == 0);
+  }();
+  return v;
+}
+
+// `alone` (if not null) receives the calls that run as single launches
+// instead of running them on `s`; `cost` (if not null) the cost per group.
 int group_batched(sth_ctx* ctx, const Plan* const* plans, int32_t n, bool serial_only, hipStream_t s,
-                  BatchGroups* groups) {
+                  BatchGroups* groups, std::vector<const Plan*>* alone = nullptr,
+                  std::map<int64_t, double>* cost = nullptr) {
   int rc = STH_OK;
   for (int32_t i = 0; i < n; ++i) {
     const Plan& pl = *plans[i];
-    if (serial_only || pl.bvariant < 0 || !pl.fn || runs_alone(pl, ctx->cus)) {
-      if (pl.fn || pl.halo)
-        if ((rc = run_plan(pl, s))) return rc;
+    if (serial_only || pl.bvariant < 0 || !pl.fn || (batch_alone_enabled() && runs_alone(pl, ctx->cus))) {
+      if (!(pl.fn || pl.halo)) continue;
+      if (alone) alone->push_back(&pl);
+      else if ((rc = run_plan(pl, s))) return rc;
       continue;
     }
+    if (cost) (*cost)[static_cast<int64_t>(pl.dtype) * 65536 + pl.bklass] += plan_cost(pl);
     if (pl.halo) {  // update_boundary! before the batched gather
       pl.halo_fn(pl.hp, s);
       STH_HIP(hipGetLastError());
@@ -865,19 +916,26 @@ int prepare_group(int64_t key, std::vector<BatchJob>& bj, const BatchKernelEntry
 
 // One step of many independent calls: calls with a batched variant go into
 // one launch per (element type, shape, reduction) after their halo refresh;
-// the rest, and calls that fill the device alone, run as single launches
-// first.  The job tables are uploaded into device slots the context owns.
+// the rest, and calls that fill the device alone, run as single launches.
+// The job tables are uploaded into device slots the context owns.  With
+// more than one launch, the launches are spread over the context's batch
+// streams (largest first, each to the least loaded stream) between a fork
+// and a join on `s`, so one launch's tail overlaps the next launch's start.
 int launch_batched(sth_ctx* ctx, const Plan* const* plans, int32_t n, bool serial_only, hipStream_t s) {
   BatchGroups groups;
-  int rc = group_batched(ctx, plans, n, serial_only, s, &groups);
+  std::vector<const Plan*> alone;
+  std::map<int64_t, double> gcost;
+  // the unbatched path keeps one launch per call, in order, on `s`
+  int rc = group_batched(ctx, plans, n, serial_only, s, &groups, serial_only ? nullptr : &alone, &gcost);
   if (rc) return rc;
-  for (auto& kv : groups) {
+  auto launch_group = [&](BatchGroups::value_type& kv, hipStream_t st) -> int {
     std::vector<BatchJob>& bj = kv.second;
     const BatchKernelEntry* ke = nullptr;
     BatchParams bp;
     int64_t blocks = 0;
-    if ((rc = prepare_group(kv.first, bj, &ke, &bp, &blocks))) return rc;
-    if (bp.total_tiles == 0) continue;
+    int r2 = prepare_group(kv.first, bj, &ke, &bp, &blocks);
+    if (r2) return r2;
+    if (bp.total_tiles == 0) return STH_OK;
     const size_t bytes = bj.size() * sizeof(BatchJob);
     std::lock_guard<std::mutex> lk(ctx->mu);
     BatchSlot* hit = nullptr;
@@ -905,18 +963,59 @@ int launch_batched(sth_ctx* ctx, const Plan* const* plans, int32_t n, bool seria
     if (!sl.done) STH_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
     if (!hit) {
       std::memcpy(sl.host, bj.data(), bytes);
-      STH_HIP(hipMemcpyAsync(sl.dev, sl.host, bytes, hipMemcpyHostToDevice, s));
+      STH_HIP(hipMemcpyAsync(sl.dev, sl.host, bytes, hipMemcpyHostToDevice, st));
       sl.content = bytes;
     } else {
       // after the slot's upload and every launch on it so far, whatever stream
       // they ran on (so an overwrite that waits for `done` waits for all)
-      STH_HIP(hipStreamWaitEvent(s, sl.done, 0));
+      STH_HIP(hipStreamWaitEvent(st, sl.done, 0));
     }
     bp.jobs = static_cast<const BatchJob*>(sl.dev);
-    ke->fn(bp, static_cast<unsigned>(blocks), s);
+    ke->fn(bp, static_cast<unsigned>(blocks), st);
     STH_HIP(hipGetLastError());
-    STH_HIP(hipEventRecord(sl.done, s));
+    STH_HIP(hipEventRecord(sl.done, st));
     sl.used = true;
+    return STH_OK;
+  };
+  // the launches, most costly first
+  struct Item {
+    const Plan* alone;
+    BatchGroups::value_type* group;
+    double cost;
+  };
+  std::vector<Item> items;
+  for (const Plan* pl : alone) items.push_back({pl, nullptr, plan_cost(*pl)});
+  for (auto& kv : groups) items.push_back({nullptr, &kv, gcost[kv.first]});
+  std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.cost > b.cost; });
+  auto run_item = [&](const Item& it, hipStream_t st) -> int {
+    return it.alone ? run_plan(*it.alone, st) : launch_group(*it.group, st);
+  };
+  const int nst = batch_streams_wanted();
+  if (items.size() <= 1 || nst <= 1) {
+    for (const Item& it : items)
+      if ((rc = run_item(it, s))) return rc;
+    return STH_OK;
+  }
+  std::lock_guard<std::mutex> fork_lock(ctx->batch_mu);
+  if ((rc = ensure_batch_streams(ctx, nst))) return rc;
+  STH_HIP(hipEventRecord(ctx->batch_fork, s));
+  double load[kBatchStreams] = {};
+  bool used[kBatchStreams] = {};
+  for (const Item& it : items) {
+    int k = 0;
+    for (int i = 1; i < nst; ++i)
+      if (load[i] < load[k]) k = i;
+    if (!used[k]) {
+      STH_HIP(hipStreamWaitEvent(ctx->batch_streams[k], ctx->batch_fork, 0));
+      used[k] = true;
+    }
+    load[k] += it.cost;
+    if ((rc = run_item(it, ctx->batch_streams[k]))) return rc;
+  }
+  for (int k = 0; k < nst; ++k) {
+    if (!used[k]) continue;
+    STH_HIP(hipEventRecord(ctx->batch_join[k], ctx->batch_streams[k]));
+    STH_HIP(hipStreamWaitEvent(s, ctx->batch_join[k], 0));
   }
   return STH_OK;
 }
@@ -1027,6 +1126,11 @@ int sth_ctx_destroy(sth_ctx* ctx) {
     if (sl.dev) (void)hipFree(sl.dev);
   }
   if (ctx->capture) (void)hipStreamDestroy(ctx->capture);
+  for (int k = 0; k < kBatchStreams; ++k) {
+    if (ctx->batch_streams[k]) (void)hipStreamDestroy(ctx->batch_streams[k]);
+    if (ctx->batch_join[k]) (void)hipEventDestroy(ctx->batch_join[k]);
+  }
+  if (ctx->batch_fork) (void)hipEventDestroy(ctx->batch_fork);
   delete ctx;
   return STH_OK;
   STH_GUARD_END
