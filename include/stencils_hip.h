@@ -352,6 +352,8 @@ typedef struct sth_server_stats {
   int64_t cell_steps;    /* completed requests' cells x steps */
   size_t hbm_in_use, hbm_peak;
   double started;        /* wall seconds since sth_server_start */
+  int64_t solo_launches;   /* launches on the solo requests' own streams */
+  int64_t fused_launches;  /* of those, launches that advance several steps */
 } sth_server_stats;
 
 int sth_server_create(sth_ctx* ctx, const sth_server_config* cfg, void* stream, sth_server** out);

@@ -1346,8 +1346,10 @@ int sth_iterate(sth_ctx* ctx, const sth_array* a, const sth_array* b, const sth_
     if ((rc = enqueue(s, units, tail))) return rc;
   } else {
     // Graphs of steady steps (starting from buffer b) are captured once per
-    // parameter set and replayed.
-    auto get_graph = [&](const char* tag, int64_t nunits, int64_t nplain, hipGraphExec_t* exec) -> int {
+    // parameter set and replayed.  The lookup and the launches happen under
+    // ctx->mu, so a cache eviction on another thread cannot destroy an
+    // executable between the two.
+    auto launch_graph = [&](const char* tag, int64_t nunits, int64_t nplain, int64_t times) -> int {
       std::string key(reinterpret_cast<const char*>(&ab), sizeof(Plan));
       key.append(reinterpret_cast<const char*>(&ba), sizeof(Plan));
       key.append(tag);
@@ -1356,7 +1358,7 @@ int sth_iterate(sth_ctx* ctx, const sth_array* a, const sth_array* b, const sth_
       std::lock_guard<std::mutex> lk(ctx->mu);
       auto it = ctx->graphs.find(key);
       if (it != ctx->graphs.end()) {
-        *exec = it->second;
+        for (int64_t i = 0; i < times; ++i) STH_HIP(hipGraphLaunch(it->second, s));
         return STH_OK;
       }
       STH_HIP(hipStreamBeginCapture(ctx->capture, hipStreamCaptureModeThreadLocal));
@@ -1377,34 +1379,20 @@ int sth_iterate(sth_ctx* ctx, const sth_array* a, const sth_array* b, const sth_
         ctx->graphs.clear();
       }
       ctx->graphs.emplace(key, x);
-      *exec = x;
+      for (int64_t i = 0; i < times; ++i) STH_HIP(hipGraphLaunch(x, s));
       return STH_OK;
     };
     if (fuse) {
       constexpr int64_t FU = 16;  // units per replayed graph
       const int64_t reps = units / FU, remu = units % FU;
-      if (reps > 0) {
-        hipGraphExec_t x;
-        if ((rc = get_graph("f", FU, 0, &x))) return rc;
-        for (int64_t i = 0; i < reps; ++i) STH_HIP(hipGraphLaunch(x, s));
-      }
-      hipGraphExec_t x;
-      if ((rc = get_graph("f", remu, tail, &x))) return rc;
-      STH_HIP(hipGraphLaunch(x, s));
+      if (reps > 0 && (rc = launch_graph("f", FU, 0, reps))) return rc;
+      if ((rc = launch_graph("f", remu, tail, 1))) return rc;
     } else {
       const int64_t U = rest >= 64 ? 16 : std::max<int64_t>(1, rest / 2);
       const int64_t G = 2 * U;
       const int64_t reps = rest / G, rem = rest % G;
-      if (reps > 0) {
-        hipGraphExec_t x;
-        if ((rc = get_graph("p", 0, G, &x))) return rc;
-        for (int64_t i = 0; i < reps; ++i) STH_HIP(hipGraphLaunch(x, s));
-      }
-      if (rem > 0) {
-        hipGraphExec_t x;
-        if ((rc = get_graph("p", 0, rem, &x))) return rc;
-        STH_HIP(hipGraphLaunch(x, s));
-      }
+      if (reps > 0 && (rc = launch_graph("p", 0, G, reps))) return rc;
+      if (rem > 0 && (rc = launch_graph("p", 0, rem, 1))) return rc;
     }
   }
   STH_HIP(hipGetLastError());

@@ -224,4 +224,50 @@ __device__ __forceinline__ int xcd_block_id(int bid, int nblocks, int remap) {
   return v;
 }
 
+// Raw buffer accesses: wave-uniform byte offset (soffset) + 32-bit per-lane
+// byte offset, one instruction per access and no 64-bit address math.
+template <typename T>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  if constexpr (sizeof(T) == 2) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0));
+  } else if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+  } else if constexpr (sizeof(T) == 8) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  } else {
+    static_assert(sizeof(T) == 16, "2-, 4-, 8- or 16-byte accesses");
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+  }
+}
+// Cache policy of stores whose data this launch never reads back: the
+// non-temporal hint (`nt`) streams them past the caches.  It pays for grids
+// larger than the 256 MB Infinity Cache (the z-marching one-step kernel at
+// 512^3: 1.05-1.3x; the 2-D two-step kernel at 16384^2: 1.03x) and costs
+// where the next launch would find the data there (Game of Life 8192^2, two
+// 67 MB buffers: 1.3x slower; the 3-D two-step kernel on f32 512^3: 1.13x
+// slower), so each kernel chooses.
+constexpr int kStoreNT = 2;
+
+// AUX is the cache policy of the store (kStoreNT: non-temporal).
+template <typename T, int AUX = 0>
+__device__ __forceinline__ void buf_store(T v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  if constexpr (sizeof(T) == 2) {
+    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, voff, soff, AUX);
+  } else if constexpr (sizeof(T) == 4) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, voff, soff, AUX);
+  } else if constexpr (sizeof(T) == 8) {
+    using V2 = unsigned int __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, voff, soff, AUX);
+  } else {
+    static_assert(sizeof(T) == 16, "2-, 4-, 8- or 16-byte accesses");
+    using V4 = unsigned int __attribute__((ext_vector_type(4)));
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V4, v), r, voff, soff, AUX);
+  }
+}
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uint32_t bytes) {
+  // gfx9 raw buffer (stride 0): accesses at byte offsets >= `bytes` are
+  // dropped (stores) or return 0 (loads); word 3 is 0x00020000
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, static_cast<int>(bytes), 0x00020000);
+}
+
 }  // namespace sth

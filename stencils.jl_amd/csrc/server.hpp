@@ -38,6 +38,7 @@ struct sth_server {
     std::vector<unsigned char> weights;
     uint64_t padval = 0;
     int64_t cells = 0;
+    size_t lead = 0;        // bytes before the parent's first element (Halo interior on 16 bytes)
     size_t pitch = 0;       // bytes per parent row in the arena
     size_t row_bytes = 0;   // bytes per parent row in the caller's layout
     int64_t rows = 0;       // parent rows (all dims but the first)
@@ -66,6 +67,9 @@ struct sth_server {
   static constexpr int kLookahead = 2;
   hipEvent_t marks[kLookahead][1 + kSoloStreams] = {};
   int64_t round_no = 0;
+  // launch counts of the solo requests (scheduler thread only; published in
+  // `stats` under `mu` after each round)
+  int64_t solo_launches = 0, fused_launches = 0;
 
   sth_ctx* ctx = nullptr;
   hipStream_t stream = nullptr;
@@ -126,7 +130,7 @@ struct sth_server {
 
   sth_array buffer_desc(const Req& q, int c) const {
     sth_array a = q.r.state;
-    a.data = arena + q.off[c];
+    a.data = arena + q.off[c] + q.lead;
     const int64_t es = sth::dtype_size(a.dtype);
     const int64_t h = a.halo;
     a.stride[0] = 1;
@@ -193,7 +197,7 @@ struct sth_server {
       // both buffers start as the state, as a SwitchingStencilArray's source
       // and dest do (src/array.jl:593-598)
       for (const size_t o : {o0, o1}) {
-        const hipError_t ce = hipMemcpy2DAsync(arena + o, q.pitch, q.r.state.data, q.row_bytes, q.row_bytes,
+        const hipError_t ce = hipMemcpy2DAsync(arena + o + q.lead, q.pitch, q.r.state.data, q.row_bytes, q.row_bytes,
                                                static_cast<size_t>(q.rows), hipMemcpyDefault, stream);
         if (ce != hipSuccess) return sth::fail(STH_EDEVICE, "copy-in: %s", hipGetErrorString(ce));
       }
@@ -254,7 +258,7 @@ struct sth_server {
       if (je == hipSuccess) je = hipStreamWaitEvent(stream, q.join, 0);
       if (je != hipSuccess) return sth::fail(STH_EDEVICE, "solo stream join: %s", hipGetErrorString(je));
     }
-    hipError_t e = hipMemcpy2DAsync(q.r.result, q.row_bytes, arena + q.off[c], q.pitch, q.row_bytes,
+    hipError_t e = hipMemcpy2DAsync(q.r.result, q.row_bytes, arena + q.off[c] + q.lead, q.pitch, q.row_bytes,
                                     static_cast<size_t>(q.rows), hipMemcpyDefault, stream);
     if (e != hipSuccess) return sth::fail(STH_EDEVICE, "copy-out: %s", hipGetErrorString(e));
     e = hipEventRecord(q.fin, stream);
@@ -293,6 +297,7 @@ struct sth_server {
       q.cur ^= 1;
       q.done += 1;
       *launches += 1;
+      solo_launches += 1;
       n -= 1;
     }
     const int64_t rem = q.r.steps - q.done;
@@ -303,12 +308,15 @@ struct sth_server {
       q.cur ^= 1;
       q.done += G;
       *launches += 1;
+      solo_launches += 1;
+      fused_launches += 1;
     }
     for (int64_t i = G * fused; i < n; ++i) {
       if (int rc = sth::run_plan(q.plan[q.cur], st)) return rc;
       q.cur ^= 1;
       q.done += 1;
       *launches += 1;
+      solo_launches += 1;
     }
     return STH_OK;
   }
@@ -410,6 +418,8 @@ struct sth_server {
       lk.lock();
       stats.launches += launches;
       stats.rounds += 1;
+      stats.solo_launches = solo_launches;
+      stats.fused_launches = fused_launches;
       if (rc) {
         fail_all(rc);
         done_cv.notify_all();
@@ -505,12 +515,16 @@ int sth_server_submit(sth_server* srv, const sth_request* q) {
   if (ndim > 1 && q->state.stride[1] != q->state.size[0] + 2 * h)
     return fail(STH_EINVAL, "request state must be dense (stride[1] = size[0] + 2 halo)");
   r->pitch = (r->row_bytes + 63) & ~static_cast<size_t>(63);
-  r->bytes_one = (r->pitch * static_cast<size_t>(rows) + 255) & ~static_cast<size_t>(255);
+  // buffers start on 256 bytes and rows and planes are 64-byte multiples, so
+  // a lead of this many bytes puts the first interior cell on 16 bytes (the
+  // vector kernels' alignment), as a Halo StencilArray's own allocation does
+  r->lead = (16 - static_cast<size_t>(h * es) % 16) % 16;
+  r->bytes_one = (r->lead + r->pitch * static_cast<size_t>(rows) + 255) & ~static_cast<size_t>(255);
   // validate the call once, against stand-in buffers of the arena layout
   {
     sth_array a = srv->buffer_desc(*r, 0), b = a;
-    a.data = reinterpret_cast<void*>(uintptr_t(1) << 20);
-    b.data = reinterpret_cast<void*>((uintptr_t(1) << 20) + 2 * r->bytes_one + 4096);
+    a.data = reinterpret_cast<void*>((uintptr_t(1) << 20) + r->lead);
+    b.data = reinterpret_cast<void*>((uintptr_t(1) << 20) + 2 * r->bytes_one + 4096 + r->lead);
     sth_stencil st = q->stencil;
     st.offsets = r->offs.data();
     st.weights = r->weights.empty() ? nullptr : r->weights.data();

@@ -174,4 +174,5 @@ class SthServerStats(ctypes.Structure):
                 ("active", ctypes.c_int64), ("pending", ctypes.c_int64),
                 ("rounds", ctypes.c_int64), ("launches", ctypes.c_int64),
                 ("cell_steps", ctypes.c_int64), ("hbm_in_use", ctypes.c_size_t),
-                ("hbm_peak", ctypes.c_size_t), ("started", ctypes.c_double)]
+                ("hbm_peak", ctypes.c_size_t), ("started", ctypes.c_double),
+                ("solo_launches", ctypes.c_int64), ("fused_launches", ctypes.c_int64)]

@@ -160,6 +160,36 @@ def test_large_requests_two_step_launches(gpu):
         assert torch.equal(c.result, ref), (S, steps)  # halo included
 
 
+def test_served_2d_requests_run_solo_two_step_kernels(gpu, oracle):
+    """2-D requests of 2^21 cells with multi-step kernels (Float32 heat,
+    Conditional; Life on a Halo-padded grid whose cells hold bytes 0..5)
+    run as solo launches, most of them advancing several steps (the Halo
+    interior sits on 16 bytes in the arena, so the vector kernels apply; the
+    first served Life step normalises the bytes); results, halo included,
+    equal the step-by-step oracle."""
+    import torch
+    dev = torch.device("cuda")
+    rng = np.random.default_rng(4)
+    H = (rng.random((2048, 1024)) * 2 - 1).astype(np.float32)
+    G = (rng.random((2048, 1024)) < 0.3).astype(np.uint8) * rng.integers(1, 6, (2048, 1024)).astype(np.uint8)
+    cases = [(sh.Laplacian(0.2), H, sh.VonNeumann(1), sh.Conditional(), 23),
+             (sh.LifeRule(), G, sh.Moore(1), sh.Halo(), 21)]
+    reqs, hosts = [], []
+    for f, A, st, pad, steps in cases:
+        t = sh.colmajor_empty(A.shape, torch.from_numpy(A[:1, :1]).dtype, dev)
+        t.copy_(torch.from_numpy(A))
+        reqs.append((f, sh.SwitchingStencilArray(t, st, boundary=sh.Wrap(), padding=pad), steps, 0.0))
+        hosts.append(sh.SwitchingStencilArray(np.asfortranarray(A), st, boundary=sh.Wrap(), padding=pad))
+    done, tags, stats = _serve(reqs, hbm_budget=1 << 30, round_steps=4)
+    assert len(done) == 2
+    assert stats["fused_launches"] >= 2 and stats["solo_launches"] < 23 + 21, stats
+    for c in done:
+        f, S, steps = tags[c.tag]
+        k = next(i for i, r in enumerate(reqs) if r[1] is S)
+        O = oracle.mapstencil_steps(f, hosts[k], steps)
+        np.testing.assert_array_equal(c.result.cpu().numpy(), O.source, err_msg=f"{f}")
+
+
 def test_invalid_request_rejected_at_submit(gpu):
     """A request the library cannot run (reach >= axis, the reference's
     ArgumentError) is rejected when submitted, through the C ABI past the

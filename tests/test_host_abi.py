@@ -55,8 +55,8 @@ int main(void) {
          offsetof(sth_request, tag));
   printf("%zu %zu %zu %zu\n", sizeof(sth_completion), offsetof(sth_completion, arrival),
          offsetof(sth_completion, finished), offsetof(sth_completion, cells));
-  printf("%zu %zu %zu\n", sizeof(sth_server_stats), offsetof(sth_server_stats, hbm_in_use),
-         offsetof(sth_server_stats, started));
+  printf("%zu %zu %zu %zu\n", sizeof(sth_server_stats), offsetof(sth_server_stats, hbm_in_use),
+         offsetof(sth_server_stats, started), offsetof(sth_server_stats, fused_launches));
   return 0;
 }''')
     exe = tmp_path / "layout"
@@ -76,7 +76,7 @@ int main(void) {
     M = _abi.SthCompletion
     assert out[6] == [ctypes.sizeof(M), M.arrival.offset, M.finished.offset, M.cells.offset]
     T = _abi.SthServerStats
-    assert out[7] == [ctypes.sizeof(T), T.hbm_in_use.offset, T.started.offset]
+    assert out[7] == [ctypes.sizeof(T), T.hbm_in_use.offset, T.started.offset, T.fused_launches.offset]
 
 
 def test_header_enums_match_python():
