@@ -196,6 +196,53 @@ int sth_map(sth_ctx* ctx, const sth_array* src, const sth_array* dst,
             int32_t padding, const sth_op* op, uint32_t flags, void* stream);
 
 /*
+ * One argument of a multi-array gather: a StencilArray (its parent, stencil,
+ * boundary and padding, as for sth_map) or a plain array, and the fold its
+ * term applies:
+ *   stencil.length >= 1: op folds the argument's stencil values (sum, mean,
+ *     minimum, maximum, kernelproduct, Laplacian, Life; result in the
+ *     argument's element type);
+ *   stencil.length == 0: the centre value, center(hood) or a plain array
+ *     read at I (src/gatherstencil.jl:112-113); `op` is ignored.  A Halo
+ *     StencilArray read this way still gets its halo refreshed, as the
+ *     reference refreshes every StencilArray argument.
+ * scaled = 1 makes the term coef * fold (coef exactly representable in the
+ * element type; float types only).
+ */
+typedef struct sth_term {
+  sth_array src;
+  sth_stencil stencil;
+  sth_op op;
+  const void* padval;  /* one value of src.dtype (Remove only) */
+  int32_t boundary;
+  int32_t padding;
+  double coef;
+  int32_t scaled;
+  int32_t reserved;
+} sth_term;
+
+#define STH_COMBINE_ADD 0 /* t_0 + t_1 + ... (left to right, each rounded) */
+#define STH_COMBINE_MUL 1 /* t_0 * t_1 * ... */
+#define STH_MAX_TERMS 8
+
+/*
+ * gatherstencil!(f, dest, source, args...) with several arguments
+ * (src/gatherstencil.jl:89-103, kernel :105-109): dest[I] = t_0 (+|*) t_1
+ * (+|*) ... with t_k the term of argument k at I, for f of the form
+ * (h_0, h_1, ...) -> g_0(h_0) + c_1 * g_1(h_1) + ...  Every argument's
+ * logical size equals dst's (else STH_EINVAL, the reference's
+ * _checksizes, :115-121); every element type equals dst's (else
+ * STH_ENOTSUP, as are term results of another type, e.g. an integer
+ * mean).  Halo arguments are refreshed first (update_boundary!, :93-95).
+ * Element types: Float32, Float64, UInt8, Int32, Int64 (integer `+` and
+ * `*` wrap; an integer sum term needs STH_OPF_INT_WRAP as for sth_map).
+ * mapstencil!(f, A::SwitchingStencilArray, B, ...) (:84-88) is this call
+ * with A's source as argument 0 and A's dest buffer as dst.
+ */
+int sth_map_multi(sth_ctx* ctx, int32_t nterms, const sth_term* terms, const sth_array* dst,
+                  int32_t combine, uint32_t flags, void* stream);
+
+/*
  * `steps` calls of mapstencil!(f, A::SwitchingStencilArray)
  * (src/gatherstencil.jl:77-83 + switch, src/array.jl:610-611), ping-ponging
  * buffers a (source) and b (dest) on device.  On return *current is 0 if

@@ -182,3 +182,63 @@ def f32_to_bf16(x: float) -> int:
 
 def bf16_to_f32(b: int) -> float:
     return lib().orc_bf16_to_f32(b)
+
+
+def _interior(parent: np.ndarray, halo: int) -> np.ndarray:
+    return parent[tuple(slice(halo, s - halo) for s in parent.shape)] if halo else parent
+
+
+def mapstencil_multi(f, args) -> np.ndarray:
+    """Reference result of gatherstencil!(f, dest, args...) for a `Terms`
+    function over host-parent StencilArrays and plain numpy arrays
+    (src/gatherstencil.jl:89-113): every StencilArray argument's halo is
+    refreshed (update_boundary!, :93-95; Halo parents change in place), then
+    per cell the terms fold left in the element type, `+` or `*` each
+    rounded (integers wrap).  A term's fold over a stencil is the single-array
+    restatement above stored in the element type -- the value `f` receives
+    from `sum(hood)` etc. -- so composing the per-term results cell by cell
+    is the reference's expression."""
+    from stencils_hip.array import AbstractStencilArray
+    from stencils_hip.boundary import Halo, Use
+    from stencils_hip.ops import Center, Scaled
+    vals = []
+    dt = None
+    for g, a in zip(f.terms, args):
+        coef = None
+        if isinstance(g, Scaled):
+            coef, g = g.coef, g.g
+        if isinstance(a, AbstractStencilArray):
+            if isinstance(g, Center):
+                if isinstance(a.padding, Halo) and not isinstance(a.boundary, Use):
+                    d, pv, _ = _parts(a)
+                    desc = _desc(a.parent(), d, a.size(), a.halo())
+                    pvb = ctypes.create_string_buffer(pv, len(pv)) if pv else None
+                    rc = lib().orc_update_boundary(ctypes.byref(desc), a.boundary.code, pvb)
+                    if rc != 0:
+                        raise ValueError(f"oracle status {rc}")
+                v = np.array(_interior(a.parent(), a.halo()), order="F")
+            else:
+                v = mapstencil(g, a)
+        else:
+            v = np.asarray(a)
+        if dt is None:
+            dt = v.dtype
+        if v.dtype != dt:
+            raise ValueError("terms of different element types")
+        if coef is not None:
+            v = dt.type(coef) * v
+        vals.append(v)
+    acc = np.array(vals[0], order="F")
+    with np.errstate(over="ignore"):
+        for v in vals[1:]:
+            acc = acc * v if f.combine == "*" else acc + v
+    return np.asfortranarray(acc.astype(dt, copy=False))
+
+
+def mapstencil_multi_switching(f, A, rest):
+    """mapstencil!(f, A::SwitchingStencilArray, rest...) on host parents
+    (src/gatherstencil.jl:84-88): A's dest interior <- f(A, rest...), then
+    the switched array."""
+    out = mapstencil_multi(f, (A,) + tuple(rest))
+    _interior(A.dest, A.halo())[...] = out
+    return A.switch()

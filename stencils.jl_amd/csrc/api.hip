@@ -22,6 +22,7 @@
 #include "life2_kernel.hpp"
 #include "lifeb_kernel.hpp"
 #include "life_kernel.hpp"
+#include "multi_kernel.hpp"
 #include "registry.hpp"
 #include "stencils_hip.h"
 
@@ -60,6 +61,10 @@ void launch_fill_jobs(BatchJob* dst, const JobChunk& c, int n, hipStream_t s) {
 
 std::vector<KernelEntry>& registry() {
   static std::vector<KernelEntry> r;
+  return r;
+}
+std::vector<MultiKernelEntry>& multi_registry() {
+  static std::vector<MultiKernelEntry> r;
   return r;
 }
 std::vector<Fused2Entry>& fused2_registry() {
@@ -914,6 +919,53 @@ int prepare_group(int64_t key, std::vector<BatchJob>& bj, const BatchKernelEntry
   return STH_OK;
 }
 
+// Runs `launch(table)` on stream `st` with `table` the device copy of `bytes`
+// bytes at `data`, held in one of the context's slots (pinned staging plus a
+// device buffer): a slot already holding the same bytes is reused (the
+// stream waits for its upload and earlier launches), else the least recently
+// used slot is refilled once its last launch has finished.
+template <class F>
+int with_device_table(sth_ctx* ctx, const void* data, size_t bytes, hipStream_t st, F&& launch) {
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  BatchSlot* hit = nullptr;
+  for (BatchSlot& c : ctx->slots)
+    if (c.used && c.content == bytes && std::memcmp(c.host, data, bytes) == 0) {
+      hit = &c;
+      break;
+    }
+  BatchSlot* lru = &ctx->slots[0];
+  for (BatchSlot& c : ctx->slots)
+    if (c.last_use < lru->last_use) lru = &c;
+  BatchSlot& sl = hit ? *hit : *lru;
+  sl.last_use = ++ctx->slot_clock;
+  if (!hit && sl.used) STH_HIP(hipEventSynchronize(sl.done));  // its last launch has finished with it
+  if (!hit && sl.bytes < bytes) {
+    if (sl.host) STH_HIP(hipHostFree(sl.host));
+    if (sl.dev) STH_HIP(hipFree(sl.dev));
+    sl.host = sl.dev = nullptr;
+    sl.bytes = 0;
+    const size_t cap = std::max<size_t>(bytes, 16384);
+    STH_HIP(hipHostMalloc(&sl.host, cap, hipHostMallocDefault));
+    STH_HIP(hipMalloc(&sl.dev, cap));
+    sl.bytes = cap;
+  }
+  if (!sl.done) STH_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+  if (!hit) {
+    std::memcpy(sl.host, data, bytes);
+    STH_HIP(hipMemcpyAsync(sl.dev, sl.host, bytes, hipMemcpyHostToDevice, st));
+    sl.content = bytes;
+  } else {
+    // after the slot's upload and every launch on it so far, whatever stream
+    // they ran on (so an overwrite that waits for `done` waits for all)
+    STH_HIP(hipStreamWaitEvent(st, sl.done, 0));
+  }
+  launch(static_cast<const void*>(sl.dev));
+  STH_HIP(hipGetLastError());
+  STH_HIP(hipEventRecord(sl.done, st));
+  sl.used = true;
+  return STH_OK;
+}
+
 // One step of many independent calls: calls with a batched variant go into
 // one launch per (element type, shape, reduction) after their halo refresh;
 // the rest, and calls that fill the device alone, run as single launches.
@@ -936,46 +988,10 @@ int launch_batched(sth_ctx* ctx, const Plan* const* plans, int32_t n, bool seria
     int r2 = prepare_group(kv.first, bj, &ke, &bp, &blocks);
     if (r2) return r2;
     if (bp.total_tiles == 0) return STH_OK;
-    const size_t bytes = bj.size() * sizeof(BatchJob);
-    std::lock_guard<std::mutex> lk(ctx->mu);
-    BatchSlot* hit = nullptr;
-    for (BatchSlot& c : ctx->slots)
-      if (c.used && c.content == bytes && std::memcmp(c.host, bj.data(), bytes) == 0) {
-        hit = &c;
-        break;
-      }
-    BatchSlot* lru = &ctx->slots[0];
-    for (BatchSlot& c : ctx->slots)
-      if (c.last_use < lru->last_use) lru = &c;
-    BatchSlot& sl = hit ? *hit : *lru;
-    sl.last_use = ++ctx->slot_clock;
-    if (!hit && sl.used) STH_HIP(hipEventSynchronize(sl.done));  // its last launch has finished with it
-    if (!hit && sl.bytes < bytes) {
-      if (sl.host) STH_HIP(hipHostFree(sl.host));
-      if (sl.dev) STH_HIP(hipFree(sl.dev));
-      sl.host = sl.dev = nullptr;
-      sl.bytes = 0;
-      const size_t cap = std::max<size_t>(bytes, 16384);
-      STH_HIP(hipHostMalloc(&sl.host, cap, hipHostMallocDefault));
-      STH_HIP(hipMalloc(&sl.dev, cap));
-      sl.bytes = cap;
-    }
-    if (!sl.done) STH_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
-    if (!hit) {
-      std::memcpy(sl.host, bj.data(), bytes);
-      STH_HIP(hipMemcpyAsync(sl.dev, sl.host, bytes, hipMemcpyHostToDevice, st));
-      sl.content = bytes;
-    } else {
-      // after the slot's upload and every launch on it so far, whatever stream
-      // they ran on (so an overwrite that waits for `done` waits for all)
-      STH_HIP(hipStreamWaitEvent(st, sl.done, 0));
-    }
-    bp.jobs = static_cast<const BatchJob*>(sl.dev);
-    ke->fn(bp, static_cast<unsigned>(blocks), st);
-    STH_HIP(hipGetLastError());
-    STH_HIP(hipEventRecord(sl.done, st));
-    sl.used = true;
-    return STH_OK;
+    return with_device_table(ctx, bj.data(), bj.size() * sizeof(BatchJob), st, [&](const void* dev) {
+      bp.jobs = static_cast<const BatchJob*>(dev);
+      ke->fn(bp, static_cast<unsigned>(blocks), st);
+    });
   };
   // the launches, most costly first
   struct Item {
@@ -1239,6 +1255,120 @@ int sth_map(sth_ctx* ctx, const sth_array* src, const sth_array* dst, const sth_
   if ((rc = tune_pick(ctx, pl, cands, group, flags, s, false, false, &idx))) return rc;
   apply_variant(pl, cands[idx]);
   if ((rc = run_plan(pl, s))) return rc;
+  if (!(flags & STH_ASYNC)) STH_HIP(hipStreamSynchronize(s));
+  return STH_OK;
+  STH_GUARD_END
+}
+
+static_assert(kMaxMultiTerms == STH_MAX_TERMS, "term table bound");
+
+int sth_map_multi(sth_ctx* ctx, int32_t nterms, const sth_term* terms, const sth_array* dst, int32_t combine,
+                  uint32_t flags, void* stream) {
+  STH_GUARD_BEGIN
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  if (nterms < 1 || nterms > kMaxMultiTerms || !terms)
+    return fail(STH_EINVAL, "multi-array gather with %d arguments (1..%d)", nterms, kMaxMultiTerms);
+  if (combine != STH_COMBINE_ADD && combine != STH_COMBINE_MUL) return fail(STH_EINVAL, "unknown combine %d", combine);
+  if ((rc = check_array(dst, "dest"))) return rc;
+  const int dt = dst->dtype;
+  if (dt != STH_F32 && dt != STH_F64 && dt != STH_U8 && dt != STH_I32 && dt != STH_I64)
+    return fail(STH_ENOTSUP, "multi-array gather over dtype %d is not accelerated", dt);
+  std::vector<MultiTerm> tbl(static_cast<size_t>(nterms));
+  std::vector<Plan> plans(static_cast<size_t>(nterms));
+  for (int32_t t = 0; t < nterms; ++t) {
+    const sth_term& a = terms[t];
+    if ((rc = check_array(&a.src, "argument"))) return rc;
+    if (a.src.ndim != dst->ndim) return fail(STH_EINVAL, "Source array sizes must match (argument %d ndims)", t);
+    for (int d = 0; d < dst->ndim; ++d)
+      if (a.src.size[d] != dst->size[d])
+        return fail(STH_EINVAL, "Source array sizes must match. Found: dim %d %lld and %lld (argument %d)", d + 1,
+                    static_cast<long long>(dst->size[d]), static_cast<long long>(a.src.size[d]), t);
+    if (a.src.dtype != dt)
+      return fail(STH_ENOTSUP, "argument %d has dtype %d, dest %d (promotion is not accelerated)", t, a.src.dtype, dt);
+    const bool centre = a.stencil.length == 0;
+    sth_stencil st = a.stencil;
+    sth_op op = a.op;
+    int32_t bc = a.boundary;
+    const int32_t zero3[3] = {0, 0, 0};
+    if (centre) {
+      // the value at I: a one-offset stencil at the centre (radius = halo, so
+      // a Halo argument keeps its refresh)
+      st = sth_stencil{};
+      st.radius = a.src.halo;
+      st.ndim = a.src.ndim;
+      st.length = 1;
+      st.offsets = zero3;
+      op = sth_op{};
+      op.kind = STH_OP_SUM;
+      if (a.padding == STH_CONDITIONAL) bc = STH_WRAP;  // nothing outside is read
+    } else if (out_dtype(dt, op, st.weights ? st.weights_dtype : -1) != dt) {
+      return fail(STH_ENOTSUP, "argument %d: op %d over dtype %d does not return the element type", t, op.kind, dt);
+    }
+    if (a.scaled) {
+      if (!is_float(dt)) return fail(STH_ENOTSUP, "a scaled term over an integer dtype promotes to Float64");
+      if (dt == STH_F32 && static_cast<double>(static_cast<float>(a.coef)) != a.coef)
+        return fail(STH_ENOTSUP, "coefficient %.17g is not a Float32 (the term would promote)", a.coef);
+    }
+    Plan& pl = plans[t];
+    if ((rc = make_plan(&a.src, dst, &st, bc, a.padval, a.padding, &op, flags | STH_GENERIC | STH_NO_FUSE, &pl)))
+      return rc;
+    MultiTerm& q = tbl[t];
+    std::memset(static_cast<void*>(&q), 0, sizeof(MultiTerm));
+    q.src = pl.p.src;
+    q.bc = pl.p.bc;
+    q.read_halo = pl.p.read_halo;
+    q.rx = pl.p.rx;
+    q.ry = pl.p.ry;
+    q.rz = pl.p.rz;
+    q.L = pl.p.L;
+    q.opk = op.kind;
+    q.direct = centre ? 1 : 0;
+    if (centre) q.src = make_devarr(a.src, true);
+    q.scaled = a.scaled ? 1 : 0;
+    q.padval = pl.p.padval;
+    q.op = pl.p.op;
+    q.coef = a.coef;
+    std::memcpy(q.offs, pl.p.offs, sizeof(q.offs));
+    std::memcpy(&q.w, &pl.p.w, sizeof(q.w));
+  }
+  // tile shape by array rank (the runtime-offset kernel's)
+  const int cfg = dst->ndim == 1 ? CFG_RT1D : (dst->ndim == 2 ? CFG_RT2D : CFG_RT3D);
+  const TileDims td = tile_dims(cfg);
+  int64_t max_box = 0;  // LDS elements of the largest staged box
+  for (const MultiTerm& q : tbl)
+    if (!q.direct)
+      max_box = std::max<int64_t>(max_box, int64_t(td.tx + 2 * q.rx) * (td.ty + 2 * q.ry) * (td.tz + 2 * q.rz));
+  const size_t lds = static_cast<size_t>(max_box) * dtype_size(dt);
+  if (lds > 65536) return fail(STH_ENOTSUP, "argument reach needs %zu bytes of LDS per tile", lds);
+  const MultiKernelEntry* ke = nullptr;
+  for (const auto& e : multi_registry())
+    if (e.dtype == dt && e.cfg == cfg) ke = &e;
+  if (!ke) return fail(STH_ENOTSUP, "no multi-array kernel for dtype %d", dt);
+  MultiParams mp{};
+  mp.dst = make_devarr(*dst, true);
+  mp.nterms = nterms;
+  mp.combine = combine;
+  const int64_t n0 = mp.dst.n0, n1 = mp.dst.n1, n2 = mp.dst.n2;
+  const int64_t tx = (n0 + td.tx - 1) / td.tx, ty = (n1 + td.ty - 1) / td.ty, tz = (n2 + td.tz - 1) / td.tz;
+  if (tx * ty * tz > (int64_t(1) << 31) - 1) return fail(STH_ENOTSUP, "array too large for one launch");
+  mp.tiles_x = static_cast<int32_t>(tx);
+  mp.tiles_y = static_cast<int32_t>(ty);
+  mp.tiles_z = static_cast<int32_t>(tz);
+  mp.xcd_remap = 1;
+  auto s = static_cast<hipStream_t>(stream);
+  // update_boundary! on every Halo argument first (src/gatherstencil.jl:93-95)
+  for (const Plan& pl : plans)
+    if (pl.halo) {
+      pl.halo_fn(pl.hp, s);
+      STH_HIP(hipGetLastError());
+    }
+  if (n0 * n1 * n2 > 0) {
+    rc = with_device_table(ctx, tbl.data(), tbl.size() * sizeof(MultiTerm), s, [&](const void* dev) {
+      ke->fn(mp, static_cast<const MultiTerm*>(dev), lds, s);
+    });
+    if (rc) return rc;
+  }
   if (!(flags & STH_ASYNC)) STH_HIP(hipStreamSynchronize(s));
   return STH_OK;
   STH_GUARD_END

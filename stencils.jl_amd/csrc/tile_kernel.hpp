@@ -221,38 +221,19 @@ __device__ __forceinline__ Raw load_raw(const DevArr& a, int64_t x, int64_t y, i
   return p[z * a.s2 + y * a.s1 + x];
 }
 
-// One output tile: stage the source box in LDS, fold, store.  `P` carries
-// the array descriptors, boundary and op parameters (MapParams for a single
-// array, BatchJob for one job of a batch); (bx, by, bz) is the tile.
-template <typename S, int OPK, class Sh, int TX, int TY, int TZ, class P>
-__device__ __forceinline__ void tile_body(const P& p, int bx, int by, int bz,
-                                          typename Staged<S>::type* __restrict__ tile) {
+// Stage the source box [X0, X0 + W) x [Y0, Y0 + H) x [Z0, Z0 + D) of `p`'s
+// array into `tile` (x fastest) with the workgroup's 256 threads, applying
+// the boundary outside the memory that may be read directly (the logical
+// array, plus the halo when the padding keeps a valid one).  (rx, ry, rz) is
+// the stencil's reach: cells beyond the reach box of the array feed no
+// stored result and are not loaded.  The caller synchronises afterwards.
+template <typename S, class P>
+__device__ __forceinline__ void stage_box(const P& p, int64_t X0, int64_t Y0, int64_t Z0, int W, int H, int D,
+                                          int rx, int ry, int rz, typename Staged<S>::type* __restrict__ tile) {
   using Raw = typename Elem<S>::raw;
-  using O = typename OutOf<S, OPK>::type;
-  using ORaw = typename Elem<O>::raw;
   constexpr int kThreads = 256;
-  static_assert(kThreads % TX == 0, "TX must divide the block");
-  constexpr int G = kThreads / TX;
-  static_assert((TY * TZ) % G == 0, "rows must split evenly over thread groups");
-  constexpr int KR = TY * TZ / G;
-
-  const int rx = Sh::kStatic ? Sh::RX : p.rx;
-  const int ry = Sh::kStatic ? Sh::RY : p.ry;
-  const int rz = Sh::kStatic ? Sh::RZ : p.rz;
-  constexpr int WS = TX + 2 * Sh::RX, HS = TY + 2 * Sh::RY, DS = TZ + 2 * Sh::RZ;
-  const int W = Sh::kStatic ? WS : TX + 2 * rx;
-  const int H = Sh::kStatic ? HS : TY + 2 * ry;
-  const int D = Sh::kStatic ? DS : TZ + 2 * rz;
   const int HW = H * W;
-
   const int64_t n0 = p.src.n0, n1 = p.src.n1, n2 = p.src.n2;
-  const int64_t X0 = static_cast<int64_t>(bx) * TX - rx;
-  const int64_t Y0 = static_cast<int64_t>(by) * TY - ry;
-  const int64_t Z0 = static_cast<int64_t>(bz) * TZ - rz;
-
-  // ---- 1. stage the source box -------------------------------------
-  // Memory that may be read directly: the logical array, plus the halo when
-  // the padding keeps a valid one.
   const int total = HW * D;
   const int64_t h0 = p.read_halo ? p.src.h0 : 0;
   const int64_t h1 = p.read_halo ? p.src.h1 : 0;
@@ -294,6 +275,39 @@ __device__ __forceinline__ void tile_body(const P& p, int bx, int by, int bz,
       tile[idx] = stage<S>(v);
     }
   }
+}
+
+// One output tile: stage the source box in LDS, fold, store.  `P` carries
+// the array descriptors, boundary and op parameters (MapParams for a single
+// array, BatchJob for one job of a batch); (bx, by, bz) is the tile.
+template <typename S, int OPK, class Sh, int TX, int TY, int TZ, class P>
+__device__ __forceinline__ void tile_body(const P& p, int bx, int by, int bz,
+                                          typename Staged<S>::type* __restrict__ tile) {
+  using Raw = typename Elem<S>::raw;
+  using O = typename OutOf<S, OPK>::type;
+  using ORaw = typename Elem<O>::raw;
+  constexpr int kThreads = 256;
+  static_assert(kThreads % TX == 0, "TX must divide the block");
+  constexpr int G = kThreads / TX;
+  static_assert((TY * TZ) % G == 0, "rows must split evenly over thread groups");
+  constexpr int KR = TY * TZ / G;
+
+  const int rx = Sh::kStatic ? Sh::RX : p.rx;
+  const int ry = Sh::kStatic ? Sh::RY : p.ry;
+  const int rz = Sh::kStatic ? Sh::RZ : p.rz;
+  constexpr int WS = TX + 2 * Sh::RX, HS = TY + 2 * Sh::RY, DS = TZ + 2 * Sh::RZ;
+  const int W = Sh::kStatic ? WS : TX + 2 * rx;
+  const int H = Sh::kStatic ? HS : TY + 2 * ry;
+  const int D = Sh::kStatic ? DS : TZ + 2 * rz;
+  const int HW = H * W;
+
+  const int64_t n0 = p.src.n0, n1 = p.src.n1, n2 = p.src.n2;
+  const int64_t X0 = static_cast<int64_t>(bx) * TX - rx;
+  const int64_t Y0 = static_cast<int64_t>(by) * TY - ry;
+  const int64_t Z0 = static_cast<int64_t>(bz) * TZ - rz;
+
+  // ---- 1. stage the source box -------------------------------------
+  stage_box<S>(p, X0, Y0, Z0, W, H, D, rx, ry, rz, tile);
   __syncthreads();
 
   // ---- 2. fold -------------------------------------------------------

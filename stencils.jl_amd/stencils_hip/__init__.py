@@ -12,8 +12,8 @@ from .boundary import Conditional, Halo, Reflect, Remove, Use, Wrap, padval
 from .gather import (BatchPlan, autotune, gatherstencil, gatherstencil_, mapstencil, mapstencil_,
                      mapstencil_batch, mapstencil_steps, result_dtype)
 from .server import Completion, Server, serve_latency_summary
-from .ops import (Laplacian, LifeRule, Unsupported, integer_fold_wraps, kernelproduct,
-                  maximum, mean, minimum, set_integer_fold_wraps, sum)
+from .ops import (Laplacian, LifeRule, Scaled, Terms, Unsupported, center, integer_fold_wraps,
+                  kernelproduct, maximum, mean, minimum, set_integer_fold_wraps, sum)
 from .stencil import (AngledCross, Annulus, BackSlash, Cardinal, Circle, Cross,
                       Diamond, ForwardSlash, Horizontal, Kernel, Moore, NamedStencil,
                       Ordinal, Positional, Rectangle, Stencil, Vertical, VonNeumann,

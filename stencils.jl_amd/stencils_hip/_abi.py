@@ -96,6 +96,26 @@ class SthJob(ctypes.Structure):
     ]
 
 
+class SthTerm(ctypes.Structure):
+    _fields_ = [
+        ("src", SthArray),
+        ("stencil", SthStencil),
+        ("op", SthOp),
+        ("padval", ctypes.c_void_p),
+        ("boundary", ctypes.c_int32),
+        ("padding", ctypes.c_int32),
+        ("coef", ctypes.c_double),
+        ("scaled", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+COMBINE_ADD = 0
+COMBINE_MUL = 1
+MAX_TERMS = 8
+MULTI_DTYPES = {F32, F64, U8, I32, I64}  # element types of sth_map_multi
+
+
 def out_dtype(src_dt: int, opkind: int, flags: int = 0, wdtype: int = -1) -> int:
     """Result element type of `op` over `src_dt` (the reference's _return_type,
     src/gatherstencil.jl:41-56), or -1 when the GPU path does not cover it."""

@@ -24,7 +24,7 @@ from .array import (AbstractStencilArray, StencilArray, SwitchingStencilArray,
                     colmajor_copy, colmajor_empty, is_device, sth_dtype_of,
                     torch_dtype, to_numpy)
 from .boundary import Conditional, Halo, Remove, Use
-from .ops import Reduction, Unsupported, integer_fold_wraps, resolve
+from .ops import Center, Reduction, Scaled, Terms, Unsupported, integer_fold_wraps, resolve
 from .stencil import Kernel, Stencil
 
 
@@ -162,6 +162,116 @@ def _map_device(f, dst, dst_halo, sa: AbstractStencilArray, src_buf, flags: int 
                          flags, _stream(dev)))
 
 
+def _term_parts(g):
+    """(fold, coef or None) of one term of a Terms function."""
+    if isinstance(g, Scaled):
+        return g.g, g.coef
+    return g, None
+
+
+def _multi_check(f: Terms, args):
+    if not isinstance(f, Terms):
+        raise Unsupported("a multi-array gather is accelerated for Terms functions only")
+    if len(f) != len(args):
+        raise TypeError(f"{f!r} takes {len(f)} arguments, got {len(args)}")
+    if len(args) > _abi.MAX_TERMS:
+        raise Unsupported(f"more than {_abi.MAX_TERMS} arguments")
+
+
+def _map_multi_device(f: Terms, dst, dst_halo, args, bufs, flags: int = 0):
+    """One sth_map_multi call: argument k is `args[k]` (a StencilArray or a
+    plain array) whose device parent is `bufs[k]`."""
+    call = _Call()
+    terms = (_abi.SthTerm * len(args))()
+    dt = sth_dtype_of(dst)
+    size = None
+    for k, (a, buf) in enumerate(zip(args, bufs)):
+        g, coef = _term_parts(f.terms[k])
+        t = terms[k]
+        if isinstance(a, AbstractStencilArray):
+            size = a.size()
+            t.src = _desc(buf, a.size(), a.halo())
+            t.boundary = a.boundary.code
+            t.padding = a.padding.code
+            pv = _padval_bytes(a) if (not isinstance(g, Center) or not isinstance(a.padding, Conditional)) else None
+            if pv is not None:
+                t.padval = ctypes.cast(call.hold(ctypes.create_string_buffer(pv, len(pv))), ctypes.c_void_p)
+            adt = a.sth_dtype()
+            if isinstance(g, Center):
+                t.stencil.length = 0
+            else:
+                t.stencil = stencil_descriptor(a.stencil, adt, call)
+                t.op = g.op(_op_flags(adt))
+        else:
+            if not isinstance(g, Center):
+                raise TypeError(f"argument {k} is a plain array (read at I): its term must be `center`, not {g!r}")
+            t.src = _desc(buf, tuple(buf.shape), 0)
+            t.boundary = _abi.WRAP
+            t.padding = _abi.CONDITIONAL
+            t.stencil.length = 0
+        if coef is not None:
+            t.scaled = 1
+            t.coef = float(coef)
+    dsc = _desc(dst, size if size is not None else tuple(dst.shape), dst_halo)
+    dev = _device_index(dst)
+    L = _lib.lib()
+    _lib.check(L.sth_map_multi(_lib.context(dev), len(args), terms, ctypes.byref(dsc), f.code(), flags,
+                               _stream(dev)))
+
+
+def _multi_parent(a):
+    return a.parent() if isinstance(a, AbstractStencilArray) else a
+
+
+def gatherstencil_multi_(f: Terms, dest, args, dest_halo: int = 0, flags: int = 0):
+    """gatherstencil!(f, dest, source, args...) with several arguments
+    (src/gatherstencil.jl:89-103): every StencilArray argument's halo is
+    refreshed, then dest[I] = f(args[0] at I, args[1] at I, ...)."""
+    _multi_check(f, args)
+    _require_backend()
+    sizes = [tuple(a.size()) if isinstance(a, AbstractStencilArray) else tuple(np.shape(a)) for a in args]
+    dsize = tuple(np.shape(dest)) if dest_halo == 0 else tuple(s - 2 * dest_halo for s in np.shape(dest))
+    for s in sizes:
+        if s != dsize:
+            raise ValueError(f"Source array sizes must match. Found: {dsize} and {s}")
+    parents = [_multi_parent(a) for a in args]
+    if all(is_device(p) for p in parents) and is_device(dest):
+        _map_multi_device(f, dest, dest_halo, args, parents, flags)
+        return dest
+    # host round trip: copy every argument in, the result and the refreshed
+    # halos back
+    torch = _torch()
+    dev = torch.cuda.current_device()
+    bufs = [p if is_device(p) else colmajor_copy(p, dev) for p in parents]
+    dbuf = dest if is_device(dest) else colmajor_copy(dest, dev)
+    _map_multi_device(f, dbuf, dest_halo, args, bufs, flags)
+    if not is_device(dest):
+        dest[...] = to_numpy(dbuf)
+    for a, p, b in zip(args, parents, bufs):
+        if not is_device(p) and isinstance(a, AbstractStencilArray) and isinstance(a.padding, Halo):
+            p[...] = to_numpy(b)
+    return dest
+
+
+def _switching_multi_step(f: Terms, A: SwitchingStencilArray, rest, flags: int = 0):
+    """mapstencil!(f, A::SwitchingStencilArray, B, ...) (src/gatherstencil.jl:84-88):
+    A's dest interior <- f(A, B, ...), then switch."""
+    _multi_check(f, (A,) + tuple(rest))
+    _require_backend()
+    torch = _torch()
+    if is_device(A.source):
+        gatherstencil_multi_(f, A.dest, (A,) + tuple(rest), A.halo(), flags)
+        return A.switch()
+    dev = torch.cuda.current_device()
+    s = colmajor_copy(A.source, dev)
+    d = colmajor_copy(A.dest, dev)
+    proxy = SwitchingStencilArray(stencil=A.stencil, boundary=A.boundary, padding=A.padding, _buffers=(s, d))
+    gatherstencil_multi_(f, d, (proxy,) + tuple(rest), A.halo(), flags)
+    A.source[...] = to_numpy(s)
+    A.dest[...] = to_numpy(d)
+    return A.switch()
+
+
 def autotune(f, dest, sa: AbstractStencilArray) -> str:
     """Time every kernel variant that can run gatherstencil!(f, dest, sa) on
     device buffers (sth_autotune), keep the fastest for this call's family,
@@ -194,10 +304,15 @@ def _require_backend():
 
 def gatherstencil_(f, dest, source=None, *args, flags: int = 0):
     """gatherstencil!(f, dest, source) or gatherstencil!(f, A::SwitchingStencilArray)."""
-    if isinstance(dest, SwitchingStencilArray) and source is None:
-        return _switching_step(f, dest, flags)
+    if isinstance(dest, SwitchingStencilArray):
+        if source is None:
+            return _switching_step(f, dest, flags)
+        return _switching_multi_step(f, dest, (source,) + args, flags)
     if args:
-        raise Unsupported("multi-array gather (extra arrays) is not accelerated")
+        if isinstance(dest, AbstractStencilArray):
+            gatherstencil_multi_(f, dest.parent(), (source,) + args, dest.halo(), flags)
+            return dest
+        return gatherstencil_multi_(f, dest, (source,) + args, 0, flags)
     if not isinstance(source, AbstractStencilArray):
         raise TypeError("source must be a StencilArray")
     resolve(f)
@@ -256,6 +371,11 @@ def gatherstencil(f, *args, boundary=None, padding=None):
     sa = args[0]
     if not isinstance(sa, AbstractStencilArray):
         raise TypeError("gatherstencil(f, A) needs a StencilArray")
+    if len(args) > 1:
+        # several arguments: the result has the arguments' element type
+        _multi_check(f, args)
+        dest = _alloc_like(sa.parent(), sa.sth_dtype(), sa.size())
+        return gatherstencil_multi_(f, dest, args)
     T = result_dtype(f, sa)
     dest = _alloc_like(sa.parent(), T, sa.size())
     return gatherstencil_(f, dest, sa, *args[1:])

@@ -81,6 +81,64 @@ class Laplacian(Reduction):
         return o
 
 
+class Center(Reduction):
+    """``center(hood)``: the value at the centre index (src/array.jl:32); for
+    a plain array argument of a multi-array gather, the value at I
+    (``_getarg(a::AbstractArray, I) = a[I]``, src/gatherstencil.jl:112)."""
+
+    def __init__(self):
+        super().__init__("center", -1)
+
+    def op(self, flags: int = 0) -> _abi.SthOp:
+        raise Unsupported("center is a term of a multi-array gather, not a reduction of its own")
+
+
+center = Center()
+
+
+class Scaled:
+    """``coef * g(hood)``: a term scaled by a constant of the element type
+    (Julia evaluates ``0.1 * sum(neighbors(h))`` as one rounded product)."""
+
+    def __init__(self, coef, g):
+        self.coef = coef
+        self.g = g if isinstance(g, Reduction) else resolve(g)
+
+    def __repr__(self):
+        return f"{self.coef!r} * {self.g!r}"
+
+
+class Terms:
+    """A function of several hoods, ``f(h_0, h_1, ...)``, of the form
+    ``g_0(h_0) + g_1(h_1) + ...`` (``combine="+"``) or the product
+    (``combine="*"``), folded left to right in the element type, each
+    operation rounded: the multi-array gather's accelerated family
+    (src/gatherstencil.jl:84-88, 105-113).  Each ``g_k`` is ``center``, a
+    reduction (``sum``, ``mean``, ``minimum``, ``maximum``,
+    ``kernelproduct``, ``Laplacian``, ``LifeRule``) or ``Scaled(coef, g)``.
+    Examples from test/array.jl:312-383:
+    ``(a, b) -> center(a) + center(b)`` is ``Terms(center, center)``;
+    ``(m, r) -> center(m) + 0.1 * sum(neighbors(r))`` is
+    ``Terms(center, Scaled(0.1, sum))``."""
+
+    def __init__(self, *gs, combine: str = "+"):
+        if not gs:
+            raise ValueError("Terms needs at least one term")
+        if combine not in ("+", "*"):
+            raise ValueError("combine is '+' or '*'")
+        self.terms = [g if isinstance(g, (Reduction, Scaled)) else resolve(g) for g in gs]
+        self.combine = combine
+
+    def __len__(self):
+        return len(self.terms)
+
+    def __repr__(self):
+        return "Terms(" + f" {self.combine} ".join(map(repr, self.terms)) + ")"
+
+    def code(self) -> int:
+        return _abi.COMBINE_MUL if self.combine == "*" else _abi.COMBINE_ADD
+
+
 _ALIASES = {
     id(builtins.sum): sum, id(np.sum): sum,
     id(np.mean): mean, id(statistics.mean): mean, id(statistics.fmean): mean,
