@@ -188,6 +188,54 @@ def bench_mean_blur(sh, torch, oracle_ok: bool):
     return out
 
 
+def bench_multi(sh, torch, n: int = 8192):
+    """Multi-array gather (sth_map_multi) on n x n Float32 fields, uniform
+    [0,1) (PCG64 seed 6): heat with a source, one step of
+    mapstencil!(Terms(Laplacian(0.2), Scaled(0.05, center)), S, Q) with S a
+    SwitchingStencilArray (VonNeumann(1), Wrap, Conditional) and Q a plain
+    array (12 B/cell: two reads, one write); and three StencilArrays
+    (Window(1) mean, Moore(1) maximum, VonNeumann(2) sum; Remove(0f0)) summed
+    (16 B/cell)."""
+    rng = np.random.Generator(np.random.PCG64(6))
+    dev = torch.device("cuda", torch.cuda.current_device())
+
+    def field():
+        t = sh.colmajor_empty((n, n), torch.float32, dev)
+        t.copy_(torch.from_numpy(rng.random((n, n), dtype=np.float32)).to(dev).T)
+        return t
+
+    def timed(fn, reps=20):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3 / reps
+
+    U, Q = field(), field()
+    S = [sh.SwitchingStencilArray(U, sh.VonNeumann(1), boundary=sh.Wrap())]
+    f = sh.Terms(sh.Laplacian(0.2), sh.Scaled(float(np.float32(0.05)), sh.center))
+
+    def step():
+        S[0] = sh.mapstencil_(f, S[0], Q, flags=sh._abi.ASYNC)
+    t_heat = timed(step)
+    A, B, C = field(), field(), field()
+    z = sh.Remove(np.float32(0))
+    args = (sh.StencilArray(A, sh.Window(1), boundary=z), sh.StencilArray(B, sh.Moore(1), boundary=z),
+            sh.StencilArray(C, sh.VonNeumann(2), boundary=z))
+    g = sh.Terms(sh.mean, sh.maximum, sh.sum)
+    out = sh.colmajor_empty((n, n), torch.float32, dev)
+    t_three = timed(lambda: sh.mapstencil_(g, out, *args, flags=sh._abi.ASYNC))
+    cells = n * n
+    return {"workload": f"coupled fields {n}x{n} Float32: heat with a source (Laplacian(0.2) + 0.05 * source), "
+                        "and three StencilArrays (mean + maximum + sum)",
+            "heat_source_us": t_heat * 1e6, "heat_source_gbs": 12 * cells / t_heat / 1e9,
+            "three_arrays_us": t_three * 1e6, "three_arrays_gbs": 16 * cells / t_three / 1e9}
+
+
 def gaussian_weights(sigma: float = 1.5, R: int = 3) -> np.ndarray:
     a = np.arange(-R, R + 1)
     W = np.exp(-(a[:, None] ** 2 + a[None, :] ** 2) / (2 * sigma ** 2))
@@ -539,7 +587,7 @@ def main():
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the secondary workloads (mean blur, Gaussian)")
     ap.add_argument("--only", choices=["life", "heat3d-step", "heat3d", "heat2d", "served", "batched", "gaussian",
-                                       "meanblur"],
+                                       "meanblur", "multi"],
                     help="run one secondary workload alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -560,7 +608,8 @@ def main():
                    "served": lambda: bench_served(sh, torch),
                    "batched": lambda: bench_batched(sh, torch),
                    "gaussian": lambda: bench_gaussian(sh, torch),
-                   "meanblur": lambda: bench_mean_blur(sh, torch, False)}[args.only]
+                   "meanblur": lambda: bench_mean_blur(sh, torch, False),
+                   "multi": lambda: bench_multi(sh, torch)}[args.only]
         print(json.dumps({args.only: run_one()}), flush=True)
         return
 
@@ -648,7 +697,8 @@ def main():
                           "batched_uneven": bench_batched(sh, torch),
                           "heat_3d_iterate": bench_heat_iterate(sh, torch),
                           "heat_2d_iterate": bench_heat_iterate_2d(sh, torch),
-                          "served_heat_3d": bench_served(sh, torch)}
+                          "served_heat_3d": bench_served(sh, torch),
+                          "coupled_fields": bench_multi(sh, torch)}
     print(json.dumps(out), flush=True)
 
 

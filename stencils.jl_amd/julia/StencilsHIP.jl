@@ -29,7 +29,7 @@ using AMDGPU: ROCArray
 import AMDGPU
 using StaticArrays: SVector
 
-export LifeRule, Laplacian, mapstencil_steps!, mapstencil_batch!, sth_available
+export LifeRule, Laplacian, Terms, Scaled, mapstencil_steps!, mapstencil_batch!, sth_available
 export Server, submit!, start!, completions!, drain!, close!
 
 const libsth = get(ENV, "STENCILS_HIP_LIB",
@@ -91,6 +91,18 @@ struct SthJob
     padval::Ptr{Cvoid}
     boundary::Int32
     padding::Int32
+end
+
+struct SthTerm
+    src::SthArray
+    stencil::SthStencil
+    op::SthOp
+    padval::Ptr{Cvoid}
+    boundary::Int32
+    padding::Int32
+    coef::Float64
+    scaled::Int32
+    reserved::Int32
 end
 
 struct SthServerConfig
@@ -161,6 +173,38 @@ function (l::Laplacian)(hood)
     return c + T(l.alpha) * (sum(neighbors(hood)) - length(hood) * c)
 end
 
+"""
+    Scaled(coef, g)
+
+The term `coef * g(hood)` of a [`Terms`] function (`coef` of the element
+type: a Float64 constant on Float32 data would promote the result).
+"""
+struct Scaled{C,G}
+    coef::C
+    g::G
+end
+
+"""
+    Terms(g_1, g_2, ...; combine=+)
+
+A function of several hoods, `(h_1, h_2, ...) -> g_1(h_1) + g_2(h_2) + ...`
+(or `*`), folded left to right: the family the multi-array gather
+accelerates (`mapstencil(f, A, B, ...)`, `mapstencil!(f, S, B, ...)`,
+src/gatherstencil.jl:84-113).  Each `g_k` is `center`, `sum`, `mean`,
+`minimum`, `maximum`, `kernelproduct`, a [`LifeRule`], a [`Laplacian`] or a
+[`Scaled`] term; a plain array argument arrives as its value at `I`, so its
+term is `center`.  On the CPU (the reference path) it evaluates exactly that.
+"""
+struct Terms{G<:Tuple,C}
+    terms::G
+    combine::C
+end
+Terms(gs...; combine=+) = Terms(gs, combine)
+_termval(::typeof(center), x::Number) = x
+_termval(g, h) = g(h)
+_termval(s::Scaled, h) = s.coef * _termval(s.g, h)
+(t::Terms)(hs...) = foldl(t.combine, map(_termval, t.terms, hs))
+
 # --------------------------------------------------------- op descriptors
 _op(::typeof(sum), T) = SthOp(0, _accflag(T) | _intflag(T), 0.0, 0, 0)
 _op(::typeof(mean), T) = SthOp(1, _accflag(T) | _intflag(T), 0.0, 0, 0)
@@ -228,7 +272,6 @@ function _array(p::ROCArray{T,N}, logical::NTuple{N,Int}, halo::Int) where {T,N}
     SthArray(Ptr{Cvoid}(UInt(pointer(p))), DTYPES[T], N, sz, sd, halo, 0)
 end
 
-_halo(A::StencilArray) = padding(A) isa Halo ? radius(stencil(A)) : 0
 
 function _stencil_desc(st::Stencil{R,N,L}) where {R,N,L}
     base = st isa AbstractKernelStencil ? stencil(st) : st
@@ -285,6 +328,88 @@ function Stencils.gatherstencil!(f::AccelOp,
     st == STH_ENOTSUP || _throw(st)
     return invoke(Stencils.gatherstencil!,
                   Tuple{Any,Any,AbstractStencilArray}, f, dest, source)
+end
+
+# ------------------------------------------------------ multi-array gather
+_halo(A::AbstractStencilArray) = padding(A) isa Halo ? radius(stencil(A)) : 0
+_devparent(A::AbstractStencilArray) = parent(A)
+_devparent(A::ROCArray) = A
+
+# One sth_term per argument, or nothing when the call is outside the family.
+function _term(g, A, ::Type{T}, keep) where T
+    coef, scaled = 0.0, Int32(0)
+    if g isa Scaled
+        T <: AbstractFloat || return nothing
+        g.coef isa T || return nothing   # the reference result would promote
+        coef, scaled, g = Float64(g.coef), Int32(1), g.g
+    end
+    eltype(A) === T || return nothing
+    if A isa ROCArray
+        g === center || return nothing
+        return SthTerm(_array(A, size(A), 0), SthStencil(0, 0, 0, 0, C_NULL, C_NULL, -1, 0),
+                       SthOp(0, 0, 0.0, 0, 0), C_NULL, Int32(1), Int32(0), coef, scaled, 0)
+    end
+    pv = _padval(boundary(A), T)
+    pv === nothing && return nothing
+    push!(keep, pv)
+    sa = _array(_devparent(A), size(A), _halo(A))
+    if g === center
+        sd = SthStencil(0, 0, 0, 0, C_NULL, C_NULL, -1, 0)
+        op = SthOp(0, 0, 0.0, 0, 0)
+    else
+        op = _op(g, T)
+        op === nothing && return nothing
+        offs, kind, r, n, l = _stencil_desc(stencil(A))
+        w = _weights(stencil(A), T)
+        push!(keep, offs, w)
+        sd = SthStencil(kind, r, n, l, pointer(offs),
+                        w === nothing ? C_NULL : Ptr{Cvoid}(pointer(w)),
+                        w === nothing ? Int32(-1) : DTYPES[eltype(w)], 0)
+    end
+    return SthTerm(sa, sd, op, Base.unsafe_convert(Ptr{Cvoid}, pv), _bc(boundary(A)), _pad(padding(A)),
+                   coef, scaled, 0)
+end
+
+function _map_multi(f::Terms, dest_parent::ROCArray{T}, dest_halo::Int, size_, args) where T
+    haskey(DTYPES, T) || return STH_ENOTSUP
+    length(args) == length(f.terms) || return STH_ENOTSUP
+    (f.combine === (+) || f.combine === (*)) || return STH_ENOTSUP
+    keep = Any[]
+    terms = SthTerm[]
+    for (g, A) in zip(f.terms, args)
+        t = _term(g, A, T, keep)
+        t === nothing && return STH_ENOTSUP
+        push!(terms, t)
+    end
+    da = _array(dest_parent, size_, dest_halo)
+    return GC.@preserve keep terms ccall((:sth_map_multi, libsth), Cint,
+        (Ptr{Cvoid}, Int32, Ptr{SthTerm}, Ref{SthArray}, Int32, UInt32, Ptr{Cvoid}),
+        _context(), Int32(length(terms)), terms, da, f.combine === (*) ? Int32(1) : Int32(0),
+        UInt32(0), _stream())
+end
+
+const DevArg = Union{ROCArray,AbstractStencilArray{<:Any,<:Any,<:Any,<:ROCArray}}
+
+# gatherstencil!(f, dest, source, args...) with device arguments
+function Stencils.gatherstencil!(f::Terms, dest::ROCArray,
+                                 source::AbstractStencilArray{<:Any,<:Any,<:Any,<:ROCArray},
+                                 arg1::DevArg, args::DevArg...)
+    st = _map_multi(f, dest, 0, size(source), (source, arg1, args...))
+    st == STH_OK && return dest
+    st == STH_ENOTSUP || _throw(st)
+    return invoke(Stencils.gatherstencil!, Tuple{Any,Any,AbstractStencilArray,Vararg{AbstractArray}},
+                  f, dest, source, arg1, args...)
+end
+
+# dest a StencilArray over a device array: mapstencil!(f, S::SwitchingStencilArray, B, ...)
+function Stencils.gatherstencil!(f::Terms, dest::StencilArray{<:Any,<:Any,<:Any,<:ROCArray},
+                                 source::AbstractStencilArray{<:Any,<:Any,<:Any,<:ROCArray},
+                                 arg1::DevArg, args::DevArg...)
+    st = _map_multi(f, parent(dest), _halo(dest), size(source), (source, arg1, args...))
+    st == STH_OK && return dest
+    st == STH_ENOTSUP || _throw(st)
+    return invoke(Stencils.gatherstencil!, Tuple{Any,Any,AbstractStencilArray,Vararg{AbstractArray}},
+                  f, dest, source, arg1, args...)
 end
 
 """
