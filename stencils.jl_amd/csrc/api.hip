@@ -1448,9 +1448,9 @@ int sth_iterate(sth_ctx* ctx, const sth_array* a, const sth_array* b, const sth_
   }
   *current = static_cast<int32_t>(steps % 2);
   if (steps == 0) return STH_OK;
-  // The first step runs eagerly (it may read data this library did not
-  // write); steps 1.. use the steady-state kernels.
-  if ((rc = run_plan(ab, s))) return rc;
+  // The first step runs the normalising kernel (it may read data this
+  // library did not write); steps 1.. use the steady-state kernels.  With
+  // graphs it is recorded at the head of the first graph launched.
   const int64_t rest = steps - 1;
   // Temporal blocking: G = fused_gens steps per launch.  A fused launch
   // writes only the newest generation, so the buffer it reads is left G - 1
@@ -1463,8 +1463,8 @@ int sth_iterate(sth_ctx* ctx, const sth_array* a, const sth_array* b, const sth_
   const bool fuse = !(flags & STH_NO_FUSE) && ab.fn_fused && ba.fn_fused && G >= 2 && rest >= 2 * G + 2;
   const int64_t units = fuse ? (rest - 2) / (2 * G) : 0;  // fused pairs
   const int64_t tail = rest - 2 * G * units;              // single steps (2 .. 2G + 1 when fused)
-  auto enqueue = [&](hipStream_t st, int64_t nunits, int64_t nplain) -> int {
-    int r2 = STH_OK;
+  auto enqueue = [&](hipStream_t st, bool first, int64_t nunits, int64_t nplain) -> int {
+    int r2 = first ? run_plan(ab, st) : STH_OK;
     for (int64_t u = 0; u < nunits && r2 == STH_OK; ++u) {
       r2 = run_fused(ba, st);
       if (r2 == STH_OK) r2 = run_fused(ab, st);
@@ -1473,16 +1473,17 @@ int sth_iterate(sth_ctx* ctx, const sth_array* a, const sth_array* b, const sth_
     return r2;
   };
   if (!(flags & STH_GRAPH)) {
-    if ((rc = enqueue(s, units, tail))) return rc;
+    if ((rc = enqueue(s, true, units, tail))) return rc;
   } else {
     // Graphs of steady steps (starting from buffer b) are captured once per
     // parameter set and replayed.  The lookup and the launches happen under
     // ctx->mu, so a cache eviction on another thread cannot destroy an
     // executable between the two.
-    auto launch_graph = [&](const char* tag, int64_t nunits, int64_t nplain, int64_t times) -> int {
+    auto launch_graph = [&](const char* tag, bool first, int64_t nunits, int64_t nplain, int64_t times) -> int {
       std::string key(reinterpret_cast<const char*>(&ab), sizeof(Plan));
       key.append(reinterpret_cast<const char*>(&ba), sizeof(Plan));
       key.append(tag);
+      key.push_back(first ? '1' : '0');
       key.append(reinterpret_cast<const char*>(&nunits), sizeof nunits);
       key.append(reinterpret_cast<const char*>(&nplain), sizeof nplain);
       std::lock_guard<std::mutex> lk(ctx->mu);
@@ -1492,7 +1493,7 @@ int sth_iterate(sth_ctx* ctx, const sth_array* a, const sth_array* b, const sth_
         return STH_OK;
       }
       STH_HIP(hipStreamBeginCapture(ctx->capture, hipStreamCaptureModeThreadLocal));
-      const int r2 = enqueue(ctx->capture, nunits, nplain);
+      const int r2 = enqueue(ctx->capture, first, nunits, nplain);
       hipGraph_t g = nullptr;
       hipError_t e = hipStreamEndCapture(ctx->capture, &g);
       if (r2) return r2;
@@ -1512,17 +1513,25 @@ int sth_iterate(sth_ctx* ctx, const sth_array* a, const sth_array* b, const sth_
       for (int64_t i = 0; i < times; ++i) STH_HIP(hipGraphLaunch(x, s));
       return STH_OK;
     };
+    // the first graph launched carries the first step; a call's graphs
+    // (head, repeated body, tail) are one graph when the call is short
     if (fuse) {
       constexpr int64_t FU = 16;  // units per replayed graph
       const int64_t reps = units / FU, remu = units % FU;
-      if (reps > 0 && (rc = launch_graph("f", FU, 0, reps))) return rc;
-      if ((rc = launch_graph("f", remu, tail, 1))) return rc;
+      if (reps > 0) {
+        if ((rc = launch_graph("f", true, FU, 0, 1))) return rc;
+        if (reps > 1 && (rc = launch_graph("f", false, FU, 0, reps - 1))) return rc;
+      }
+      if ((rc = launch_graph("f", reps == 0, remu, tail, 1))) return rc;
     } else {
       const int64_t U = rest >= 64 ? 16 : std::max<int64_t>(1, rest / 2);
       const int64_t G = 2 * U;
       const int64_t reps = rest / G, rem = rest % G;
-      if (reps > 0 && (rc = launch_graph("p", 0, G, reps))) return rc;
-      if (rem > 0 && (rc = launch_graph("p", 0, rem, 1))) return rc;
+      if (reps > 0) {
+        if ((rc = launch_graph("p", true, 0, G, 1))) return rc;
+        if (reps > 1 && (rc = launch_graph("p", false, 0, G, reps - 1))) return rc;
+      }
+      if ((reps == 0 || rem > 0) && (rc = launch_graph("p", reps == 0, 0, rem, 1))) return rc;
     }
   }
   STH_HIP(hipGetLastError());

@@ -133,6 +133,9 @@ def set_variant(pattern: str = "", device: int = 0):
 
 def context(device: int = 0):
     """The library context of `device`, created on first use."""
+    c = _ctx.get(device)
+    if c is not None:
+        return c
     import torch
     if not torch.cuda.is_available():
         raise BackendUnavailable("no HIP device is visible; the GPU path has no CPU fallback")

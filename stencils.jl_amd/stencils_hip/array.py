@@ -254,8 +254,15 @@ class SwitchingStencilArray(AbstractStencilArray):
         return self.source
 
     def switch(self) -> "SwitchingStencilArray":
-        return SwitchingStencilArray(stencil=self.stencil, boundary=self.boundary,
-                                     padding=self.padding, _buffers=(self.dest, self.source))
+        # the buffers were checked when this array was made; the switched
+        # array takes over the library-call descriptors with the roles swapped
+        out = object.__new__(SwitchingStencilArray)
+        out.source, out.dest = self.dest, self.source
+        out.stencil, out.boundary, out.padding = self.stencil, self.boundary, self.padding
+        c = self.__dict__.get("_sth_args")
+        if c is not None:
+            out._sth_args = c[0](c)
+        return out
 
     def __repr__(self):
         return (f"SwitchingStencilArray(size={self.size()}, stencil={self.stencil!r}, "

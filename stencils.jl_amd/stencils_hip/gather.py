@@ -84,11 +84,30 @@ def _weights(st: Stencil, src_dt: int):
     return np.ascontiguousarray(w), wdt
 
 
+# Descriptors of stencils already described, per (stencil object, element
+# type): the entry keeps the stencil alive, so its id is not reused while
+# cached, and keeps the offset / weight arrays the descriptor points to.
+_STENCIL_DESCS: dict = {}
+
+
 def stencil_descriptor(st: Stencil, src_dt: int, call: _Call) -> _abi.SthStencil:
+    hit = _STENCIL_DESCS.get((id(st), src_dt))
+    if hit is not None and hit[0] is st:
+        return hit[1]
+    d, arrays = _build_stencil_descriptor(st, src_dt)
+    for a in arrays:
+        call.hold(a)
+    if len(_STENCIL_DESCS) >= 256:
+        _STENCIL_DESCS.clear()
+    _STENCIL_DESCS[(id(st), src_dt)] = (st, d, arrays)
+    return d
+
+
+def _build_stencil_descriptor(st: Stencil, src_dt: int):
+    """(descriptor, the arrays it points to)."""
     if not st.gpu_supported():
         raise Unsupported(f"{st!r} is not supported on the GPU path")
     offs = np.ascontiguousarray(np.array(st.offsets(), dtype=np.int32).reshape(-1))
-    call.hold(offs)
     d = _abi.SthStencil()
     d.kind = st.kind
     d.radius = st.radius()
@@ -97,13 +116,12 @@ def stencil_descriptor(st: Stencil, src_dt: int, call: _Call) -> _abi.SthStencil
     d.offsets = offs.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
     w, wdt = _weights(st, src_dt)
     if w is not None:
-        call.hold(w)
         d.weights = w.ctypes.data
         d.weights_dtype = wdt
-    else:
-        d.weights = None
-        d.weights_dtype = -1
-    return d
+        return d, (offs, w)
+    d.weights = None
+    d.weights_dtype = -1
+    return d, (offs,)
 
 
 def _op_flags(src_dt: int) -> int:
@@ -133,6 +151,16 @@ def _device_index(x) -> int:
 
 def _stream(dev: int):
     return ctypes.c_void_p(_torch().cuda.current_stream(dev).cuda_stream)
+
+
+def _raw_stream(dev: int):
+    """The current stream of `dev` as a hipStream_t (torch's raw getter when
+    it has one: no Stream object per call)."""
+    torch = _torch()
+    get = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+    if get is None:
+        return _stream(dev)
+    return ctypes.c_void_p(get(dev))
 
 
 def _desc(x, shape, halo) -> _abi.SthArray:
@@ -296,10 +324,17 @@ def autotune(f, dest, sa: AbstractStencilArray) -> str:
     return name.value.decode()
 
 
+_BACKEND_OK = False
+
+
 def _require_backend():
+    global _BACKEND_OK
+    if _BACKEND_OK:
+        return
     torch = _torch()
     if not torch.cuda.is_available():
         raise _lib.BackendUnavailable("no HIP device is visible; the GPU path has no CPU fallback")
+    _BACKEND_OK = True
 
 
 def gatherstencil_(f, dest, source=None, *args, flags: int = 0):
@@ -405,20 +440,19 @@ def _switching_step(f, A: SwitchingStencilArray, flags: int = 0):
     return A.switch()
 
 
-def mapstencil_steps(f, A: SwitchingStencilArray, steps: int, graph: bool = True,
-                     flags: int = 0):
-    """`steps` calls of mapstencil!(f, A) on device in one library call
-    (sth_iterate); with `graph` the steps replay as a captured hipGraph.
-    Returns the array switched as the last call of the loop would return it."""
-    red = resolve(f)
-    _require_backend()
-    if result_dtype(f, A) != A.sth_dtype():
+def _swap_iterate_args(c):
+    """The cached sth_iterate arguments of the switched array."""
+    a, b, dev, ctx, da, db, st, op, pvbuf, keep = c[6]
+    return c[:6] + ((b, a, dev, ctx, db, da, st, op, pvbuf, keep),)
+
+
+def _iterate_args(red, A: SwitchingStencilArray, dev: int, copy: bool):
+    """Device buffers and ABI descriptors of sth_iterate over A's buffers
+    (copied to the device first when `copy`)."""
+    if result_dtype(red, A) != A.sth_dtype():
         raise ValueError("the result type differs from the array's element type")
-    torch = _torch()
-    host = not is_device(A.source)
-    dev = torch.cuda.current_device() if host else _device_index(A.source)
-    a = colmajor_copy(A.source, dev) if host else A.source
-    b = colmajor_copy(A.dest, dev) if host else A.dest
+    a = colmajor_copy(A.source, dev) if copy else A.source
+    b = colmajor_copy(A.dest, dev) if copy else A.dest
     dt = A.sth_dtype()
     call = _Call()
     st = stencil_descriptor(A.stencil, dt, call)
@@ -428,13 +462,36 @@ def mapstencil_steps(f, A: SwitchingStencilArray, steps: int, graph: bool = True
     size = A.size()
     da = _desc(a, size, A.halo())
     db = _desc(b, size, A.halo())
+    return a, b, dev, _lib.context(dev), da, db, st, op, pvbuf, call
+
+
+def mapstencil_steps(f, A: SwitchingStencilArray, steps: int, graph: bool = True,
+                     flags: int = 0):
+    """`steps` calls of mapstencil!(f, A) on device in one library call
+    (sth_iterate); with `graph` the steps replay as a captured hipGraph.
+    Returns the array switched as the last call of the loop would return it."""
+    red = resolve(f)
+    _require_backend()
+    host = not is_device(A.source)
+    if host:
+        args = _iterate_args(red, A, _torch().cuda.current_device(), copy=True)
+    else:
+        # device buffers: the descriptors ride on the array (and, roles
+        # swapped, on the arrays switch() returns), valid while its stencil,
+        # boundary, padding and the function are the ones they were made for
+        c = A.__dict__.get("_sth_args")
+        if (c is not None and c[1] is red and c[2] is A.stencil and c[3] is A.boundary and c[4] is A.padding
+                and c[5] == integer_fold_wraps() and c[6][0] is A.source and c[6][1] is A.dest):
+            args = c[6]
+        else:
+            args = _iterate_args(red, A, _device_index(A.source), copy=False)
+            A._sth_args = (_swap_iterate_args, red, A.stencil, A.boundary, A.padding, integer_fold_wraps(), args)
+    a, b, dev, ctx, da, db, st, op, pvbuf, _keep = args
     cur = ctypes.c_int32(0)
-    L = _lib.lib()
-    ctx = _lib.context(dev)
     fl = flags | (_abi.GRAPH if graph else 0)
-    _lib.check(L.sth_iterate(ctx, ctypes.byref(da), ctypes.byref(db), ctypes.byref(st),
-                             A.boundary.code, pvbuf, A.padding.code, ctypes.byref(op),
-                             int(steps), fl, ctypes.byref(cur), _stream(dev)))
+    _lib.check(_lib.lib().sth_iterate(ctx, ctypes.byref(da), ctypes.byref(db), ctypes.byref(st),
+                                      A.boundary.code, pvbuf, A.padding.code, ctypes.byref(op),
+                                      int(steps), fl, ctypes.byref(cur), _raw_stream(dev)))
     if host:
         A.source[...] = to_numpy(a)
         A.dest[...] = to_numpy(b)
