@@ -102,6 +102,31 @@ def test_parity_f32_2d(gpu, oracle, st, bname):
         np.testing.assert_array_equal(g, o)
 
 
+REMAINING_SHAPES = [sh.Diamond(2), sh.AngledCross(2), sh.ForwardSlash(2), sh.BackSlash(3), sh.Vertical(2),
+                    sh.Horizontal(3), sh.Cardinal(2), sh.Cross(3), sh.Diamond(1, 3), sh.Vertical(1, 3),
+                    sh.Horizontal(1, 3)]
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+@pytest.mark.parametrize("st", REMAINING_SHAPES, ids=repr)
+def test_parity_remaining_shapes(gpu, oracle, st, dt):
+    """The shapes outside the build-time catalogue (runtime offsets) and
+    distance-weighted kernels Kernel(f, stencil) (src/stencils/kernel.jl:
+    98-109), every boundary and padding, bit-identical to the oracle."""
+    shape = (29, 23) if st.N == 2 else (13, 11, 9)
+    A = _np_input(shape, dt, 31)
+    K = sh.Kernel(lambda d: dt(1.0 / (1.0 + d)), st)
+    for bname in BOUNDS:
+        bc = BOUNDS[bname](dt)
+        for pad in PADS.values():
+            for f in (sh.sum, sh.mean, sh.minimum, sh.maximum):
+                g, o, gp, op_ = _both(f, A, st, bc, pad, oracle)
+                np.testing.assert_array_equal(g, o, err_msg=f"{bname} {pad} {f}")
+                np.testing.assert_array_equal(gp, op_)
+            g, o, _, _ = _both(sh.kernelproduct, A, K, bc, pad, oracle)
+            np.testing.assert_array_equal(g, o, err_msg=f"{bname} {pad} kernelproduct")
+
+
 @pytest.fixture
 def int_wrap():
     """Integer sum / mean with the element type's wrapping fold stated by the

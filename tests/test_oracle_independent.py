@@ -272,3 +272,40 @@ def test_host_declines_unpinned_integer_folds():
                 assert declined == (L >= 2 and dt not in _abi.FLOAT_DTYPES and dt != _abi.BOOL and
                                     ((op == _abi.OP_SUM and _abi.DTYPE_SIZE[dt] < 8) or op == _abi.OP_MEAN))
                 assert not _abi.int_fold_declined(dt, op, _abi.OPF_INT_WRAP, L)
+
+
+@pytest.mark.parametrize("R", [1, 2, 3])
+def test_remaining_shape_tables(R):
+    """Offset tables of the shapes the reference's tests do not enumerate,
+    against closed forms from their definitions' docstrings and geometry
+    (src/stencils/shapes.jl): counts, symmetry, the fold order
+    (CartesianIndices order: dim 1 fastest), and the centre where it
+    belongs."""
+    def order_ok(offs):
+        keys = [tuple(reversed(o)) for o in offs]
+        return keys == sorted(keys)
+
+    cases = {
+        sh.Diamond(R): 2 * R * R + 2 * R + 1,          # the L1 ball in 2-D
+        sh.Diamond(R, 3): (2 * R + 1) * (2 * R * R + 2 * R + 3) // 3,
+        sh.ForwardSlash(R): 2 * R + 1, sh.BackSlash(R): 2 * R + 1,   # "contains 2R+1 neighbors"
+        sh.AngledCross(R): 4 * R + 1,                  # both diagonals, one shared centre
+        sh.Vertical(R): 2 * R + 1, sh.Horizontal(R): 2 * R + 1,
+        sh.Vertical(R, 3): (2 * R + 1) ** 2, sh.Horizontal(R, 3): (2 * R + 1) ** 2,
+        sh.Cross(R): 4 * R + 1,
+    }
+    for st, n in cases.items():
+        offs = [tuple(o) for o in st.offsets()]
+        assert len(offs) == n == len(set(offs)), (st, len(offs), n)
+        assert order_ok(offs), st
+        assert (0,) * st.N in offs, st
+        assert sorted(tuple(-v for v in o) for o in offs) == sorted(offs), st  # point symmetric
+        assert max(max(abs(v) for v in o) for o in offs) == R
+    # which axis the bars run along: Vertical varies dim 1 (dim 2 fixed at 0),
+    # Horizontal varies dim 2, and the slashes are the two diagonals
+    assert all(o[1] == 0 for o in sh.Vertical(R).offsets())
+    assert all(o[0] == 0 for o in sh.Horizontal(R).offsets())
+    assert all(o[1] == -o[0] for o in sh.ForwardSlash(R).offsets())
+    assert all(o[1] == o[0] for o in sh.BackSlash(R).offsets())
+    assert set(map(tuple, sh.AngledCross(R).offsets())) == \
+        set(map(tuple, sh.ForwardSlash(R).offsets())) | set(map(tuple, sh.BackSlash(R).offsets()))
