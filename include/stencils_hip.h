@@ -243,6 +243,45 @@ int sth_map_multi(sth_ctx* ctx, int32_t nterms, const sth_term* terms, const sth
                   int32_t combine, uint32_t flags, void* stream);
 
 /*
+ * scatterstencil!(f, op, dest, source, args...) (src/scatterstencil.jl:37-47,
+ * 2-D loop :51-75, targets :78-114): every source cell I computes one value
+ * per stencil offset k and folds it into dest at I + offset_k (mapped by the
+ * source's boundary: Wrap wraps, Reflect reflects once, Remove and Use skip
+ * targets outside), dest[J] = op(dest[J], v).  The values (`f`) come from a
+ * fixed family; A is the source, B the optional second StencilArray (same
+ * offsets), c the coefficient (exactly representable in the element type):
+ *   STH_SCATTER_CONST          v = c                         map(_ -> c, neighbors(a))
+ *   STH_SCATTER_CENTER         v = A[I]                      map(_ -> center(a), ...)
+ *   STH_SCATTER_SCALED         v = A[I] * c
+ *   STH_SCATTER_DOWNHILL       d = B[I] - B_k; v = d > 0 ? A[I] * c : 0
+ *   STH_SCATTER_DOWNHILL_FLOW  d = B[I] - B_k; v = d > 0 ? A[I] * d * c : 0
+ * (B_k: B's neighbour k of I under B's own boundary and padding).  Each dest
+ * cell receives its values in the reference's order -- passes over source
+ * columns j = p, p + S, ... (S = 2R + 1) for p = 1..S, rows ascending, offsets
+ * in order -- so `+` rounds exactly as the reference's loop does.  2-D
+ * Float32 / Float64; op STH_COMBINE_ADD, STH_COMBINE_MAX or STH_COMBINE_MIN
+ * (Julia max / min).  STH_SCATTER_ZERO_DEST starts from zero instead of
+ * dest's values (the SwitchingStencilArray forms clear dest first, :118-136).
+ * Halo arguments are refreshed first.  A stencil whose reach exceeds its
+ * radius R gets STH_ENOTSUP (the reference's passes would overlap).
+ */
+typedef struct sth_scatter_rule {
+  int32_t kind;     /* STH_SCATTER_* */
+  int32_t combine;  /* STH_COMBINE_ADD / STH_COMBINE_MAX / STH_COMBINE_MIN */
+  double coef;
+} sth_scatter_rule;
+#define STH_SCATTER_CONST 0
+#define STH_SCATTER_CENTER 1
+#define STH_SCATTER_SCALED 2
+#define STH_SCATTER_DOWNHILL 3
+#define STH_SCATTER_DOWNHILL_FLOW 4
+#define STH_COMBINE_MAX 2 /* Julia max */
+#define STH_COMBINE_MIN 3 /* Julia min */
+#define STH_SCATTER_ZERO_DEST 64u
+int sth_scatter(sth_ctx* ctx, const sth_term* source, const sth_term* aux, const sth_array* dst,
+                const sth_scatter_rule* rule, uint32_t flags, void* stream);
+
+/*
  * `steps` calls of mapstencil!(f, A::SwitchingStencilArray)
  * (src/gatherstencil.jl:77-83 + switch, src/array.jl:610-611), ping-ponging
  * buffers a (source) and b (dest) on device.  On return *current is 0 if

@@ -242,3 +242,33 @@ def mapstencil_multi_switching(f, A, rest):
     out = mapstencil_multi(f, (A,) + tuple(rest))
     _interior(A.dest, A.halo())[...] = out
     return A.switch()
+
+
+def scatterstencil(f, op, dest: np.ndarray, source, aux=None, zero_dest: bool = False) -> np.ndarray:
+    """Reference result of scatterstencil!(f, op, dest, source[, aux]) for a
+    `Scatter` rule on host arrays (src/scatterstencil.jl:37-114), written
+    into `dest` in place (Halo parents refreshed in place)."""
+    from stencils_hip.scatter import combine_code
+    L = lib()
+    L.orc_scatter.restype = ctypes.c_int
+    dt = source.sth_dtype()
+    offs = np.ascontiguousarray(np.asarray(source.stencil.offsets(), dtype=np.int32).reshape(-1))
+    src = _desc(source.parent(), dt, source.size(), source.halo())
+    _, pv, _ = _parts(source)
+    pvb = ctypes.create_string_buffer(pv, len(pv)) if pv else None
+    dsc = _desc(dest, dt, dest.shape, 0)
+    if aux is not None:
+        adesc = _desc(aux.parent(), aux.sth_dtype(), aux.size(), aux.halo())
+        _, apv, _ = _parts(aux)
+        apvb = ctypes.create_string_buffer(apv, len(apv)) if apv else None
+        aref, abc, apad = ctypes.byref(adesc), aux.boundary.code, aux.padding.code
+    else:
+        aref, apvb, abc, apad = None, None, 0, 0
+    rc = L.orc_scatter(ctypes.byref(src), ctypes.c_int(len(source.stencil)),
+                       offs.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ctypes.c_int(source.stencil.radius()),
+                       ctypes.c_int(source.boundary.code), pvb, ctypes.c_int(source.padding.code), aref,
+                       ctypes.c_int(abc), apvb, ctypes.c_int(apad), ctypes.c_int(f.kind), ctypes.c_double(float(f.coef)),
+                       ctypes.c_int(combine_code(op)), ctypes.byref(dsc), ctypes.c_int(1 if zero_dest else 0))
+    if rc != 0:
+        raise ValueError(f"oracle status {rc}")
+    return dest

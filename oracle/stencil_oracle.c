@@ -588,6 +588,94 @@ int orc_iterate(const sth_array* a, const sth_array* b, int L, int st_ndim,
   return STH_OK;
 }
 
+/*
+ * scatterstencil!(f, op, dest, source, args...) for the fixed value rules
+ * of sth_scatter, 2-D, in the reference's loop order
+ * (src/scatterstencil.jl:37-47, 51-75): update_boundary! on the source and
+ * the second array; passes p = 1..S (S = 2R + 1) over source columns
+ * j = p, p + S, ... (1-based), rows i ascending, offsets k in order; target
+ * (i, j) + offset_k mapped by the source's boundary (:78-114: Wrap mod1,
+ * Reflect once, Remove / Use skip), dest[t] = op(dest[t], v_k).  `aux` is
+ * the second StencilArray (NULL if the rule does not read one).  Float32 /
+ * Float64.  zero_dest clears dest first (the SwitchingStencilArray forms).
+ */
+static int64_t scat_map(int64_t p, int64_t n, int bc) {
+  if (p >= 0 && p < n) return p;
+  if (bc == STH_WRAP) return ((p % n) + n) % n;
+  if (bc == STH_REFLECT) return p < 0 ? -p : 2 * n - 2 - p;
+  return -1;
+}
+
+int orc_scatter(const sth_array* src, int L, const int32_t* offsets, int R, int bc, const void* padval,
+                int padding, const sth_array* aux, int aux_bc, const void* aux_padval, int aux_padding,
+                int kind, double coef, int combine, const sth_array* dst, int zero_dest) {
+  if (src->ndim != 2 || dst->ndim != 2 || L < 1 || R < 0) return STH_EINVAL;
+  for (int d = 0; d < 2; ++d)
+    if (src->size[d] != dst->size[d] || (aux && aux->size[d] != dst->size[d])) return STH_EINVAL;
+  const int dt = dst->dtype;
+  if ((dt != STH_F32 && dt != STH_F64) || src->dtype != dt || (aux && aux->dtype != dt)) return STH_ENOTSUP;
+  if (padding != STH_CONDITIONAL) {
+    int rc = orc_update_boundary(src, bc, padval);
+    if (rc) return rc;
+  }
+  job_t* bj = NULL;
+  if (aux) {
+    if (aux_padding != STH_CONDITIONAL) {
+      int rc = orc_update_boundary(aux, aux_bc, aux_padval);
+      if (rc) return rc;
+    }
+    bj = (job_t*)calloc(1, sizeof(job_t));
+    if (!bj) return STH_ENOMEM;
+    bj->src = aux;
+    bj->bc = aux_bc;
+    bj->padval = aux_padval;
+    bj->halo_mode = aux_padding != STH_CONDITIONAL;
+  }
+  const int64_t ny = dst->size[0], nx = dst->size[1];
+  const int64_t S = 2 * R + 1;
+  if (zero_dest)
+    for (int64_t j = 0; j < nx; ++j)
+      for (int64_t i = 0; i < ny; ++i) memset((char*)elem_ptr(dst, i, j, 0), 0, (size_t)dtype_size(dt));
+  for (int64_t pass = 0; pass < S; ++pass)
+    for (int64_t j = pass; j < nx; j += S)
+      for (int64_t i = 0; i < ny; ++i) {
+        const double a = load_f64(elem_ptr(src, i, j, 0), dt);
+        const double b = aux ? load_f64(elem_ptr(aux, i, j, 0), dt) : 0.0;
+        for (int k = 0; k < L; ++k) {
+          const int oi = offsets[2 * k], oj = offsets[2 * k + 1];
+          const int64_t ti = scat_map(i + oi, ny, bc), tj = scat_map(j + oj, nx, bc);
+          if (ti < 0 || tj < 0) continue;
+          char* t = (char*)elem_ptr(dst, ti, tj, 0);
+          if (dt == STH_F64) {
+            double v = coef;
+            if (kind == STH_SCATTER_CENTER) v = a;
+            else if (kind == STH_SCATTER_SCALED) v = a * coef;
+            else if (kind >= STH_SCATTER_DOWNHILL) {
+              const char* q = fetch(bj, i + oi, j + oj, 0);
+              const double dz = b - load_f64(q ? q : (const char*)aux_padval, dt);
+              v = dz > 0 ? (kind == STH_SCATTER_DOWNHILL ? a * coef : (a * dz) * coef) : 0.0;
+            }
+            double* o = (double*)t;
+            *o = combine == STH_COMBINE_MAX ? jmax_d(*o, v) : combine == STH_COMBINE_MIN ? jmin_d(*o, v) : *o + v;
+          } else {
+            const float af = (float)a, bf = (float)b, cf = (float)coef;
+            float v = cf;
+            if (kind == STH_SCATTER_CENTER) v = af;
+            else if (kind == STH_SCATTER_SCALED) v = af * cf;
+            else if (kind >= STH_SCATTER_DOWNHILL) {
+              const char* q = fetch(bj, i + oi, j + oj, 0);
+              const float dz = bf - load_f32(q ? q : (const char*)aux_padval, dt);
+              v = dz > 0 ? (kind == STH_SCATTER_DOWNHILL ? af * cf : (af * dz) * cf) : 0.0f;
+            }
+            float* o = (float*)t;
+            *o = combine == STH_COMBINE_MAX ? jmax_f(*o, v) : combine == STH_COMBINE_MIN ? jmin_f(*o, v) : *o + v;
+          }
+        }
+      }
+  free(bj);
+  return STH_OK;
+}
+
 /* Rounding helpers exported for the tests. */
 uint16_t orc_f32_to_bf16(float f) { return f32_to_bf16(f); }
 uint16_t orc_f32_to_f16(float f) { return f32_to_f16(f); }

@@ -24,6 +24,7 @@
 #include "life_kernel.hpp"
 #include "multi_kernel.hpp"
 #include "registry.hpp"
+#include "scatter_kernel.hpp"
 #include "stencils_hip.h"
 
 namespace sth {
@@ -1368,6 +1369,98 @@ int sth_map_multi(sth_ctx* ctx, int32_t nterms, const sth_term* terms, const sth
       ke->fn(mp, static_cast<const MultiTerm*>(dev), lds, s);
     });
     if (rc) return rc;
+  }
+  if (!(flags & STH_ASYNC)) STH_HIP(hipStreamSynchronize(s));
+  return STH_OK;
+  STH_GUARD_END
+}
+
+int sth_scatter(sth_ctx* ctx, const sth_term* source, const sth_term* aux, const sth_array* dst,
+                const sth_scatter_rule* rule, uint32_t flags, void* stream) {
+  STH_GUARD_BEGIN
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  if (!source || !rule) return fail(STH_EINVAL, "source or rule is NULL");
+  if ((rc = check_array(dst, "dest"))) return rc;
+  const int dt = dst->dtype;
+  if (dt != STH_F32 && dt != STH_F64) return fail(STH_ENOTSUP, "scatter over dtype %d is not accelerated", dt);
+  if (dst->ndim != 2 || source->src.ndim != 2)
+    return fail(STH_ENOTSUP, "scatterstencil! is defined for 2-D arrays (src/scatterstencil.jl:51)");
+  if (rule->kind < STH_SCATTER_CONST || rule->kind > STH_SCATTER_DOWNHILL_FLOW)
+    return fail(STH_EINVAL, "unknown scatter rule %d", rule->kind);
+  if (rule->combine != STH_COMBINE_ADD && rule->combine != STH_COMBINE_MAX && rule->combine != STH_COMBINE_MIN)
+    return fail(STH_ENOTSUP, "scatter op %d is not accelerated", rule->combine);
+  const bool needs_aux = rule->kind >= STH_SCATTER_DOWNHILL;
+  if (needs_aux && !aux) return fail(STH_EINVAL, "the downhill rules need the second array");
+  if (dt == STH_F32 && static_cast<double>(static_cast<float>(rule->coef)) != rule->coef)
+    return fail(STH_ENOTSUP, "coefficient %.17g is not a Float32 (the values would promote)", rule->coef);
+  const sth_stencil& st = source->stencil;
+  if (st.length < 1 || st.length > kScatterMaxL || !st.offsets || st.ndim != 2)
+    return fail(STH_ENOTSUP, "scatter stencil with %d offsets in %d dims", st.length, st.ndim);
+  const int R = st.radius;
+  if (R < 0 || R > kScatterMaxR) return fail(STH_ENOTSUP, "scatter radius %d (max %d)", R, kScatterMaxR);
+  for (int k = 0; k < st.length; ++k)
+    for (int d = 0; d < 2; ++d)
+      if (std::abs(st.offsets[2 * k + d]) > R)
+        return fail(STH_ENOTSUP, "stencil reach beyond its radius %d: the reference's column passes would overlap", R);
+  // validate the arguments as gathers (sizes, boundary, padding, radius)
+  sth_op sum_op{};
+  sum_op.kind = STH_OP_SUM;
+  Plan ps, pa;
+  if ((rc = make_plan(&source->src, dst, &st, source->boundary, source->padval, source->padding, &sum_op,
+                      flags | STH_GENERIC | STH_NO_FUSE, &ps)))
+    return rc;
+  if (source->src.dtype != dt) return fail(STH_ENOTSUP, "source and dest element types differ");
+  if (needs_aux) {
+    if (aux->src.dtype != dt) return fail(STH_ENOTSUP, "the second array's element type differs");
+    if (aux->stencil.length != st.length || !aux->stencil.offsets ||
+        std::memcmp(aux->stencil.offsets, st.offsets, sizeof(int32_t) * 2 * st.length) != 0)
+      return fail(STH_ENOTSUP, "the second array's stencil differs from the source's");
+    if ((rc = make_plan(&aux->src, dst, &aux->stencil, aux->boundary, aux->padval, aux->padding, &sum_op,
+                        flags | STH_GENERIC | STH_NO_FUSE, &pa)))
+      return rc;
+  }
+  ScatterParams p;
+  std::memset(static_cast<void*>(&p), 0, sizeof p);
+  p.src = ps.p.src;
+  p.dst = make_devarr(*dst, true);
+  p.bc = source->boundary;
+  if (needs_aux) {
+    p.aux = pa.p.src;
+    p.aux_bc = pa.p.bc;
+    p.aux_read_halo = pa.p.read_halo;
+    p.aux_padval = pa.p.padval;
+    p.has_aux = 1;
+  }
+  p.kind = rule->kind;
+  p.combine = rule->combine;
+  p.zero_dest = (flags & STH_SCATTER_ZERO_DEST) ? 1 : 0;
+  p.R = R;
+  p.S = 2 * R + 1;
+  p.L = st.length;
+  p.coef = rule->coef;
+  std::memset(p.rev, -1, sizeof p.rev);
+  for (int k = 0; k < st.length; ++k) {
+    p.offs[k][0] = static_cast<int16_t>(st.offsets[2 * k]);
+    p.offs[k][1] = static_cast<int16_t>(st.offsets[2 * k + 1]);
+    int8_t& r = p.rev[(st.offsets[2 * k + 1] + R) * p.S + (st.offsets[2 * k] + R)];
+    if (r >= 0) p.general = 1;  // a repeated offset: keep every k in order
+    r = static_cast<int8_t>(k);
+  }
+  const int64_t tx = (p.dst.n0 + 63) / 64, ty = (p.dst.n1 + 3) / 4;
+  if (tx * ty > (int64_t(1) << 31) - 1) return fail(STH_ENOTSUP, "array too large for one launch");
+  p.tiles_x = static_cast<int32_t>(tx);
+  p.tiles_y = static_cast<int32_t>(ty);
+  auto s = static_cast<hipStream_t>(stream);
+  // update_boundary! on the source and on a StencilArray argument first
+  // (src/scatterstencil.jl:41-43)
+  if (ps.halo) ps.halo_fn(ps.hp, s);
+  if (needs_aux && pa.halo) pa.halo_fn(pa.hp, s);
+  STH_HIP(hipGetLastError());
+  if (tx * ty > 0) {
+    if (dt == STH_F32) launch_scatter<float>(p, s);
+    else launch_scatter<double>(p, s);
+    STH_HIP(hipGetLastError());
   }
   if (!(flags & STH_ASYNC)) STH_HIP(hipStreamSynchronize(s));
   return STH_OK;

@@ -110,8 +110,15 @@ class SthTerm(ctypes.Structure):
     ]
 
 
+class SthScatterRule(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("combine", ctypes.c_int32), ("coef", ctypes.c_double)]
+
+
+SCATTER_ZERO_DEST = 64
 COMBINE_ADD = 0
 COMBINE_MUL = 1
+COMBINE_MAX = 2
+COMBINE_MIN = 3
 MAX_TERMS = 8
 MULTI_DTYPES = {F32, F64, U8, I32, I64}  # element types of sth_map_multi
 

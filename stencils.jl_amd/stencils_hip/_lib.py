@@ -41,6 +41,9 @@ EXPORTS = {
     "sth_map_multi": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(_abi.SthTerm),
                                      ctypes.POINTER(_abi.SthArray), ctypes.c_int32, ctypes.c_uint32,
                                      ctypes.c_void_p]),
+    "sth_scatter": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_abi.SthTerm), ctypes.POINTER(_abi.SthTerm),
+                                   ctypes.POINTER(_abi.SthArray), ctypes.POINTER(_abi.SthScatterRule), ctypes.c_uint32,
+                                   ctypes.c_void_p]),
     "sth_map_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(_abi.SthJob),
                                      ctypes.c_uint32, ctypes.c_void_p]),
     "sth_map_batch_static": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(_abi.SthJob),

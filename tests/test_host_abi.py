@@ -57,6 +57,7 @@ int main(void) {
          offsetof(sth_completion, finished), offsetof(sth_completion, cells));
   printf("%zu %zu %zu %zu\n", sizeof(sth_server_stats), offsetof(sth_server_stats, hbm_in_use),
          offsetof(sth_server_stats, started), offsetof(sth_server_stats, fused_launches));
+  printf("%zu %zu\n", sizeof(sth_scatter_rule), offsetof(sth_scatter_rule, coef));
   printf("%zu %zu %zu %zu %zu %zu\n", sizeof(sth_term), offsetof(sth_term, stencil), offsetof(sth_term, op),
          offsetof(sth_term, padval), offsetof(sth_term, coef), offsetof(sth_term, scaled));
   return 0;
@@ -79,8 +80,10 @@ int main(void) {
     assert out[6] == [ctypes.sizeof(M), M.arrival.offset, M.finished.offset, M.cells.offset]
     T = _abi.SthServerStats
     assert out[7] == [ctypes.sizeof(T), T.hbm_in_use.offset, T.started.offset, T.fused_launches.offset]
+    SR = _abi.SthScatterRule
+    assert out[8] == [ctypes.sizeof(SR), SR.coef.offset]
     M2 = _abi.SthTerm
-    assert out[8] == [ctypes.sizeof(M2), M2.stencil.offset, M2.op.offset, M2.padval.offset, M2.coef.offset,
+    assert out[9] == [ctypes.sizeof(M2), M2.stencil.offset, M2.op.offset, M2.padval.offset, M2.coef.offset,
                       M2.scaled.offset]
 
 
