@@ -236,6 +236,52 @@ def bench_multi(sh, torch, n: int = 8192):
             "three_arrays_us": t_three * 1e6, "three_arrays_gbs": 16 * cells / t_three / 1e9}
 
 
+def bench_scatter(sh, torch, n: int = 8192, oracle_ok: bool = False):
+    """scatterstencil! (sth_scatter) on n x n Float64 fields (PCG64 seed 7):
+    water flowing downhill, Scatter.downhill_flow(0.1) with + over Moore(1),
+    Remove(0.0), the elevation a second StencilArray; dest starts at zero.
+    Algorithmic bytes: water and elevation read, dest written (24 B/cell)."""
+    rng = np.random.Generator(np.random.PCG64(7))
+    dev = torch.device("cuda", torch.cuda.current_device())
+
+    def field(scale):
+        t = sh.colmajor_empty((n, n), torch.float64, dev)
+        t.copy_(torch.from_numpy(rng.random((n, n)) * scale).to(dev).T)
+        return t
+
+    water, dem, dest = field(1.0), field(10.0), field(0.0)
+    f = sh.Scatter.downhill_flow(0.1)
+    w = sh.StencilArray(water, sh.Moore(1), boundary=sh.Remove(0.0))
+    z = sh.StencilArray(dem, sh.Moore(1), boundary=sh.Remove(0.0))
+
+    def run():
+        sh.scatterstencil_(f, "+", dest, w, z, flags=sh._abi.ASYNC)
+    run()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 10
+    e0.record()
+    for _ in range(reps):
+        run()
+    e1.record()
+    torch.cuda.synchronize()
+    t = e0.elapsed_time(e1) / 1e3 / reps
+    cells = n * n
+    out = {"workload": f"scatter {n}x{n} Float64: downhill flow 0.1, +, Moore(1), Remove(0.0), elevation field",
+           "kernel_us": t * 1e6, "cells_per_s": cells / t, "gbs": 24 * cells / t / 1e9}
+    if oracle_ok:
+        import oracle
+        m = 1024  # a bounded sample: the reference's loop on one thread
+        A = np.asfortranarray(rng.random((m, m)))
+        B = np.asfortranarray(rng.random((m, m)) * 10)
+        D = np.zeros((m, m), order="F")
+        t0 = time.perf_counter()
+        oracle.scatterstencil(f, "+", D, sh.StencilArray(A, sh.Moore(1), boundary=sh.Remove(0.0)),
+                              sh.StencilArray(B, sh.Moore(1), boundary=sh.Remove(0.0)))
+        out["cpu_oracle_cells_per_s_1_thread"] = m * m / (time.perf_counter() - t0)
+    return out
+
+
 def gaussian_weights(sigma: float = 1.5, R: int = 3) -> np.ndarray:
     a = np.arange(-R, R + 1)
     W = np.exp(-(a[:, None] ** 2 + a[None, :] ** 2) / (2 * sigma ** 2))
@@ -587,7 +633,7 @@ def main():
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the secondary workloads (mean blur, Gaussian)")
     ap.add_argument("--only", choices=["life", "heat3d-step", "heat3d", "heat2d", "served", "batched", "gaussian",
-                                       "meanblur", "multi"],
+                                       "meanblur", "multi", "scatter"],
                     help="run one secondary workload alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -609,7 +655,8 @@ def main():
                    "batched": lambda: bench_batched(sh, torch),
                    "gaussian": lambda: bench_gaussian(sh, torch),
                    "meanblur": lambda: bench_mean_blur(sh, torch, False),
-                   "multi": lambda: bench_multi(sh, torch)}[args.only]
+                   "multi": lambda: bench_multi(sh, torch),
+                   "scatter": lambda: bench_scatter(sh, torch, oracle_ok=True)}[args.only]
         print(json.dumps({args.only: run_one()}), flush=True)
         return
 
@@ -698,7 +745,8 @@ def main():
                           "heat_3d_iterate": bench_heat_iterate(sh, torch),
                           "heat_2d_iterate": bench_heat_iterate_2d(sh, torch),
                           "served_heat_3d": bench_served(sh, torch),
-                          "coupled_fields": bench_multi(sh, torch)}
+                          "coupled_fields": bench_multi(sh, torch),
+                          "scatter_downhill": bench_scatter(sh, torch)}
     print(json.dumps(out), flush=True)
 
 

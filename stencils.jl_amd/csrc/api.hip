@@ -1399,6 +1399,8 @@ int sth_scatter(sth_ctx* ctx, const sth_term* source, const sth_term* aux, const
     return fail(STH_ENOTSUP, "scatter stencil with %d offsets in %d dims", st.length, st.ndim);
   const int R = st.radius;
   if (R < 0 || R > kScatterMaxR) return fail(STH_ENOTSUP, "scatter radius %d (max %d)", R, kScatterMaxR);
+  if (dst->size[0] >= (int64_t(1) << 30) || dst->size[1] >= (int64_t(1) << 30))
+    return fail(STH_ENOTSUP, "scatter axis longer than 2^30 cells");
   for (int k = 0; k < st.length; ++k)
     for (int d = 0; d < 2; ++d)
       if (std::abs(st.offsets[2 * k + d]) > R)
@@ -1439,16 +1441,56 @@ int sth_scatter(sth_ctx* ctx, const sth_term* source, const sth_term* aux, const
   p.S = 2 * R + 1;
   p.L = st.length;
   p.coef = rule->coef;
-  std::memset(p.rev, -1, sizeof p.rev);
-  for (int k = 0; k < st.length; ++k) {
-    p.offs[k][0] = static_cast<int16_t>(st.offsets[2 * k]);
-    p.offs[k][1] = static_cast<int16_t>(st.offsets[2 * k + 1]);
-    int8_t& r = p.rev[(st.offsets[2 * k + 1] + R) * p.S + (st.offsets[2 * k] + R)];
-    if (r >= 0) p.general = 1;  // a repeated offset: keep every k in order
-    r = static_cast<int8_t>(k);
+  // device table: the offsets in fold order; then per dest-column residue r
+  // the order a cell receives them from the cells d - offset: the sender
+  // sits in source column dj - b (b = the offset's dim-2 part), whose pass
+  // is its residue modulo S; within a pass rows ascend (dim-1 part
+  // descending), then k.  The order depends on d only through dj mod S.
+  // Listed once as packed offsets and once as LDS deltas of the senders.
+  const int L = st.length;
+  auto pack = [&](int k) {
+    return static_cast<int32_t>((static_cast<uint32_t>(st.offsets[2 * k]) & 0xffffu) |
+                                (static_cast<uint32_t>(st.offsets[2 * k + 1]) << 16));
+  };
+  std::vector<int32_t> tbl(static_cast<size_t>(scatter_table_len(L, p.S)));
+  for (int k = 0; k < L; ++k) tbl[k] = pack(k);
+  const int W = kScatterTX + 2 * R;
+  for (int r = 0; r < p.S; ++r) {
+    std::vector<int> ks(static_cast<size_t>(L));
+    for (int k = 0; k < L; ++k) ks[k] = k;
+    auto pass = [&](int k) { return ((r - st.offsets[2 * k + 1]) % p.S + p.S) % p.S; };
+    std::stable_sort(ks.begin(), ks.end(), [&](int x, int y) {
+      if (pass(x) != pass(y)) return pass(x) < pass(y);
+      return st.offsets[2 * x] > st.offsets[2 * y];
+    });
+    for (int t = 0; t < L; ++t) {
+      tbl[static_cast<size_t>(L) * (1 + r) + t] = pack(ks[t]);
+      tbl[static_cast<size_t>(L) * (1 + p.S + r) + t] = -(st.offsets[2 * ks[t] + 1] * W + st.offsets[2 * ks[t]]);
+    }
   }
-  const int64_t tx = (p.dst.n0 + 63) / 64, ty = (p.dst.n1 + 3) / 4;
-  if (tx * ty > (int64_t(1) << 31) - 1) return fail(STH_ENOTSUP, "array too large for one launch");
+  const int64_t tx = (p.dst.n0 + kScatterTX - 1) / kScatterTX, ty = (p.dst.n1 + kScatterTY - 1) / kScatterTY;
+  // the coordinates of each axis with a wrapped or reflected preimage
+  // (Wrap: within R - 1 of the low edge or R of the high one; Reflect:
+  // 1..R from either edge), as the kernel's scatter_preimages finds them
+  auto band = [&](int64_t n, int32_t* out) {
+    int cnt = 0;
+    for (int64_t x = 0; x < n; ++x) {
+      if (x >= 2 * R + 2 && x < n - 2 * R - 2) x = n - 2 * R - 2;
+      bool nontrivial = false;
+      if (p.bc == STH_WRAP) nontrivial = x - n >= -R || x + n <= n - 1 + R;
+      else if (p.bc == STH_REFLECT) nontrivial = (x > 0 && x <= R) || (x < n - 1 && 2 * n - 2 - x <= n - 1 + R);
+      if (nontrivial) out[cnt++] = static_cast<int32_t>(x);
+    }
+    return cnt;
+  };
+  p.nbi = band(p.dst.n0, p.band_i);
+  p.nbj = band(p.dst.n1, p.band_j);
+  const int64_t gen = p.nbj * p.dst.n0 + p.nbi * (p.dst.n1 - p.nbj);
+  const int64_t gen_blocks = (gen + 255) / 256;
+  if (gen > (int64_t(1) << 30) || tx * ty + gen_blocks > (int64_t(1) << 31) - 1)
+    return fail(STH_ENOTSUP, "array too large for one launch");
+  p.gen_cells = static_cast<int32_t>(gen);
+  p.gen_blocks = static_cast<int32_t>(gen_blocks);
   p.tiles_x = static_cast<int32_t>(tx);
   p.tiles_y = static_cast<int32_t>(ty);
   auto s = static_cast<hipStream_t>(stream);
@@ -1458,9 +1500,11 @@ int sth_scatter(sth_ctx* ctx, const sth_term* source, const sth_term* aux, const
   if (needs_aux && pa.halo) pa.halo_fn(pa.hp, s);
   STH_HIP(hipGetLastError());
   if (tx * ty > 0) {
-    if (dt == STH_F32) launch_scatter<float>(p, s);
-    else launch_scatter<double>(p, s);
-    STH_HIP(hipGetLastError());
+    rc = with_device_table(ctx, tbl.data(), tbl.size() * sizeof(int32_t), s, [&](const void* dev) {
+      if (dt == STH_F32) launch_scatter<float>(p, static_cast<const int32_t*>(dev), s);
+      else launch_scatter<double>(p, static_cast<const int32_t*>(dev), s);
+    });
+    if (rc) return rc;
   }
   if (!(flags & STH_ASYNC)) STH_HIP(hipStreamSynchronize(s));
   return STH_OK;

@@ -90,6 +90,30 @@ def test_random_against_oracle(gpu, oracle, dt, shape):
         np.testing.assert_array_equal(_host(gsa.parent()), osa.parent(), err_msg=msg)
 
 
+@pytest.mark.parametrize("st", [sh.Moore(3), sh.Moore(5), sh.Diamond(7)], ids=["moore3", "moore5", "diamond7"])
+@pytest.mark.parametrize("bc", BOUNDS, ids=["remove", "wrap", "reflect"])
+def test_large_radius_against_oracle(gpu, oracle, st, bc):
+    """Radii 3 to 7 over several tiles in both axes: band cells R deep on
+    every side under Wrap and Reflect, and order tables longer than the 1024
+    entries each workgroup loads alongside its box (Moore(5): 120 offsets,
+    Diamond(7): 113, 11 and 15 residues)."""
+    rng = np.random.default_rng(len(st.offsets()))
+    shape = (200, 150)
+    for f, op in [(Scatter.downhill_flow(0.7), "+"), (Scatter.center(), max), (Scatter.scaled(0.1), "+")]:
+        A = np.asfortranarray(rng.random(shape))
+        B = np.asfortranarray(rng.random(shape))
+        D = np.asfortranarray(rng.random(shape))
+        gsa = sh.StencilArray(_dev(A), st, boundary=bc)
+        osa = sh.StencilArray(A.copy(order="F"), st, boundary=bc)
+        gaux = [sh.StencilArray(_dev(B), st, boundary=sh.Wrap())] if f.needs_aux() else []
+        oaux = sh.StencilArray(B.copy(order="F"), st, boundary=sh.Wrap()) if f.needs_aux() else None
+        gd = _dev(D)
+        od = D.copy(order="F")
+        sh.scatterstencil_(f, op, gd, gsa, *gaux)
+        oracle.scatterstencil(f, op, od, osa, oaux)
+        np.testing.assert_array_equal(_host(gd), od, err_msg=f"{f} {op} {st} {bc}")
+
+
 def test_switching_forms(gpu, oracle):
     """scatterstencil!(f, op, A::SwitchingStencilArray) clears dest and
     scatters A's source (Conditional); the explicit-source form scatters
