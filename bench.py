@@ -282,6 +282,40 @@ def bench_scatter(sh, torch, n: int = 8192, oracle_ok: bool = False):
     return out
 
 
+def bench_runtime(sh, torch, n: int = 8192):
+    """The runtime-offset tile kernel (user offset lists and integer folds) on
+    n x n arrays (PCG64 seed 9), one gatherstencil! per call, against the
+    catalogue streaming kernel on the same size: Float32 sum over a
+    six-offset Positional stencil (Wrap), Int64 sum over Moore(1)
+    (Remove(0)), UInt8 maximum over Moore(2) (Wrap), and the catalogue
+    Float32 sum over Moore(1) (Wrap).  Algorithmic bytes: one read and one
+    write of the element type per cell."""
+    rng = np.random.Generator(np.random.PCG64(9))
+    dev = torch.device("cuda", torch.cuda.current_device())
+    cases = [
+        ("f32 sum Positional((-2,0),(0,-1),(0,0),(1,1),(2,-2),(-1,2)) Wrap", torch.float32,
+         sh.Positional((-2, 0), (0, -1), (0, 0), (1, 1), (2, -2), (-1, 2)), sh.Wrap(), sh.sum),
+        ("i64 sum Moore(1) Remove(0)", torch.int64, sh.Moore(1), sh.Remove(np.int64(0)), sh.sum),
+        ("u8 maximum Moore(2) Wrap", torch.uint8, sh.Moore(2), sh.Wrap(), sh.maximum),
+        ("f32 sum Moore(1) Wrap (catalogue streaming kernel)", torch.float32, sh.Moore(1), sh.Wrap(), sh.sum),
+    ]
+    out = {"workload": f"runtime-offset tile kernel, {n}x{n}", "cases": {}}
+    for name, dt, st, bc, fold in cases:
+        t = sh.colmajor_empty((n, n), dt, dev)
+        if dt.is_floating_point:
+            t.copy_(torch.from_numpy(rng.random((n, n), dtype=np.float32)).to(dev).T)
+        else:
+            t.copy_(torch.from_numpy(rng.integers(0, 100, (n, n))).to(dt).to(dev).T)
+        sa = sh.StencilArray(t, st, boundary=bc)
+        dest = sh.colmajor_empty((n, n), dt, dev)
+        per_call = graph_time(torch, lambda: sh.mapstencil_(fold, dest, sa, flags=sh._abi.ASYNC), 10)
+        es = torch.empty((), dtype=dt).element_size()
+        out["cases"][name] = {"kernel_us": per_call * 1e6, "gbs": 2 * es * n * n / per_call / 1e9}
+        del t, dest, sa
+    out["tuning"] = sh._lib.tuning_table()
+    return out
+
+
 def gaussian_weights(sigma: float = 1.5, R: int = 3) -> np.ndarray:
     a = np.arange(-R, R + 1)
     W = np.exp(-(a[:, None] ** 2 + a[None, :] ** 2) / (2 * sigma ** 2))
@@ -633,7 +667,7 @@ def main():
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the secondary workloads (mean blur, Gaussian)")
     ap.add_argument("--only", choices=["life", "heat3d-step", "heat3d", "heat2d", "served", "batched", "gaussian",
-                                       "meanblur", "multi", "scatter"],
+                                       "meanblur", "multi", "scatter", "runtime"],
                     help="run one secondary workload alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -656,7 +690,8 @@ def main():
                    "gaussian": lambda: bench_gaussian(sh, torch),
                    "meanblur": lambda: bench_mean_blur(sh, torch, False),
                    "multi": lambda: bench_multi(sh, torch),
-                   "scatter": lambda: bench_scatter(sh, torch, oracle_ok=True)}[args.only]
+                   "scatter": lambda: bench_scatter(sh, torch, oracle_ok=True),
+                   "runtime": lambda: bench_runtime(sh, torch)}[args.only]
         print(json.dumps({args.only: run_one()}), flush=True)
         return
 

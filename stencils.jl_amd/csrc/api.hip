@@ -16,6 +16,7 @@
 
 #include "batch_kernel.hpp"
 #include "halo_kernel.hpp"
+#include "jit.hpp"
 #include "zmarch2_kernel.hpp"
 #include "zmarch1_kernel.hpp"
 #include "ymarch2_kernel.hpp"
@@ -136,6 +137,7 @@ struct Plan {
   MapParams p;
   LaunchFn fn;
   LaunchFn fn_steady;  // valid when the source was written by the previous step
+  hipFunction_t jit;   // a runtime-compiled specialisation (jit.hpp) instead of fn
   const char* kname;
   int32_t halo;
   int32_t bvariant;  // case of the batched kernel for this call, -1 if none
@@ -154,6 +156,7 @@ struct Plan {
 struct Variant {
   LaunchFn fn = nullptr;
   LaunchFn fn_steady = nullptr;
+  hipFunction_t jit = nullptr;
   const char* name = "";
   int32_t tiles_x = 0, tiles_y = 0, tiles_z = 0;
   int32_t lpr_shift = 6;  // streaming kernels: log2 of the lanes per row
@@ -171,12 +174,16 @@ int lpr_shift_for(int64_t n0, int v) {
 void apply_variant(Plan& pl, const Variant& v) {
   pl.fn = v.fn;
   pl.fn_steady = v.fn_steady;
+  pl.jit = v.jit;
   pl.kname = v.name;
   pl.p.tiles_x = v.tiles_x;
   pl.p.tiles_y = v.tiles_y;
   pl.p.tiles_z = v.tiles_z;
   pl.p.lpr_shift = v.lpr_shift;
-  if (pl.p.src.n0 * pl.p.src.n1 * pl.p.src.n2 == 0) pl.fn = pl.fn_steady = nullptr;  // nothing to launch
+  if (pl.p.src.n0 * pl.p.src.n1 * pl.p.src.n2 == 0) {  // nothing to launch
+    pl.fn = pl.fn_steady = nullptr;
+    pl.jit = nullptr;
+  }
 }
 
 DevArr make_devarr(const sth_array& a, bool with_halo) {
@@ -513,6 +520,35 @@ int make_plan(const sth_array* src, const sth_array* dst, const sth_stencil* st,
           break;
         }
       if (!pick) return fail(STH_ENOTSUP, "no kernel for dtype %d op %d", sdt, op->kind);
+      // a runtime-compiled streaming kernel for this shape, ahead of the
+      // runtime-offset kernel (which stays the tuner's alternative)
+      if (!(flags & STH_GENERIC) && jit_wanted(n[0] * n[1] * n[2]) && src->ndim >= 2 && op->kind != STH_OP_LIFE &&
+          std::max(reach[0], std::max(reach[1], reach[2])) <= kJitMaxReach && n[2] <= (int64_t(1) << 30)) {
+        const int es = dtype_size(sdt);
+        const int rmax = std::max(reach[0], std::max(reach[1], reach[2]));
+        int v = 16 / es;
+        if (es == 2 && (rmax >= 2 || reach[2] > 0)) v = 4;
+        if (es == 1) v = 8;  // 16 one-byte cells per lane take 140+ VGPRs
+        const int ky = (reach[2] > 0 || rmax <= 2) ? 4 : 2;
+        std::vector<int8_t> o8(3 * L);
+        for (int i = 0; i < 3 * L; ++i) o8[i] = static_cast<int8_t>(offs3[i]);
+        std::string why;
+        const JitKernel* jk = jit_stream_kernel(sdt, op->kind, L, o8.data(), reach[0], reach[1], reach[2], v, ky, &why);
+        if (jk) {
+          Variant jv;
+          jv.jit = jk->fn;
+          jv.name = jk->name.c_str();
+          jv.lpr_shift = lpr_shift_for(n[0], v);
+          const int64_t lpr = int64_t(1) << jv.lpr_shift, rpw = 64 / lpr;
+          const int64_t jtx = (n[0] + lpr * v - 1) / (lpr * v), jty = (n[1] + 4 * ky * rpw - 1) / (4 * ky * rpw);
+          if (jtx * jty * n[2] <= (int64_t(1) << 31) - 1) {
+            jv.tiles_x = static_cast<int32_t>(jtx);
+            jv.tiles_y = static_cast<int32_t>(jty);
+            jv.tiles_z = static_cast<int32_t>(n[2]);
+            cands.push_back(jv);
+          }
+        }
+      }
       for (int k = 0; k < L; ++k)
         p.offs[k] = (offs3[3 * k] & 0xff) | ((offs3[3 * k + 1] & 0xff) << 8) | ((offs3[3 * k + 2] & 0xff) << 16);
       const TileDims td = tile_dims(pick->cfg);
@@ -531,6 +567,11 @@ int make_plan(const sth_array* src, const sth_array* dst, const sth_stencil* st,
       v.tiles_y = static_cast<int32_t>(ty);
       v.tiles_z = static_cast<int32_t>(tz);
       cands.push_back(v);
+      if (cands.size() > 1) {  // a specialisation and the runtime-offset kernel: the tuner picks
+        const char* slash = std::strchr(cands[0].name, '/');
+        group.assign(cands[0].name, slash ? static_cast<size_t>(slash - cands[0].name) : std::strlen(cands[0].name));
+        group += bucket;
+      }
     }
   }
   apply_variant(pl, cands[0]);
@@ -639,8 +680,12 @@ int make_plan(const sth_array* src, const sth_array* dst, const sth_stencil* st,
 
 int run_plan(const Plan& pl, hipStream_t s, bool steady = false) {
   if (pl.halo) pl.halo_fn(pl.hp, s);
-  const LaunchFn f = steady ? pl.fn_steady : pl.fn;
-  if (f) f(pl.p, s);
+  if (pl.jit) {
+    STH_HIP(jit_launch(pl.jit, pl.p, s));
+  } else {
+    const LaunchFn f = steady ? pl.fn_steady : pl.fn;
+    if (f) f(pl.p, s);
+  }
   STH_HIP(hipGetLastError());
   return STH_OK;
 }
@@ -862,8 +907,8 @@ int group_batched(sth_ctx* ctx, const Plan* const* plans, int32_t n, bool serial
   int rc = STH_OK;
   for (int32_t i = 0; i < n; ++i) {
     const Plan& pl = *plans[i];
-    if (serial_only || pl.bvariant < 0 || !pl.fn || (batch_alone_enabled() && runs_alone(pl, ctx->cus))) {
-      if (!(pl.fn || pl.halo)) continue;
+    if (serial_only || pl.bvariant < 0 || !(pl.fn || pl.jit) || (batch_alone_enabled() && runs_alone(pl, ctx->cus))) {
+      if (!(pl.fn || pl.jit || pl.halo)) continue;
       if (alone) alone->push_back(&pl);
       else if ((rc = run_plan(pl, s))) return rc;
       continue;
@@ -1486,7 +1531,7 @@ int sth_scatter(sth_ctx* ctx, const sth_term* source, const sth_term* aux, const
   p.nbi = band(p.dst.n0, p.band_i);
   p.nbj = band(p.dst.n1, p.band_j);
   const int64_t gen = p.nbj * p.dst.n0 + p.nbi * (p.dst.n1 - p.nbj);
-  const int64_t gen_blocks = (gen + 255) / 256;
+  const int64_t gen_blocks = (((gen + 255) / 256) + 7) & ~int64_t(7);  // a multiple of 8: tiles start on XCD 0
   if (gen > (int64_t(1) << 30) || tx * ty + gen_blocks > (int64_t(1) << 31) - 1)
     return fail(STH_ENOTSUP, "array too large for one launch");
   p.gen_cells = static_cast<int32_t>(gen);

@@ -298,7 +298,11 @@ __global__ __launch_bounds__(256) void scatter_kernel(ScatterParams p, const int
     if (g < p.gen_cells) scatter_band_cell<T, KIND, COMB>(p, tbl, g);
     return;
   }
-  const int tile = static_cast<int>(blockIdx.x) - p.gen_blocks;
+  // Workgroups go to the 8 XCDs round robin, each XCD with its own L2:
+  // XCD k takes the k-th contiguous run of tiles, so neighbouring tiles that
+  // share halo rows and cache lines share an L2 (gen_blocks is a multiple
+  // of 8, so the tile index mod 8 is the XCD).
+  const int tile = xcd_block_id(static_cast<int>(blockIdx.x) - p.gen_blocks, p.tiles_x * p.tiles_y, 1);
   const int n0 = static_cast<int>(p.dst.n0), n1 = static_cast<int>(p.dst.n1);
   const int di0 = (tile % p.tiles_x) * kScatterTX;
   const int dj0 = (tile / p.tiles_x) * kScatterTY;
