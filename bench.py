@@ -239,8 +239,9 @@ def bench_multi(sh, torch, n: int = 8192):
 def bench_scatter(sh, torch, n: int = 8192, oracle_ok: bool = False):
     """scatterstencil! (sth_scatter) on n x n Float64 fields (PCG64 seed 7):
     water flowing downhill, Scatter.downhill_flow(0.1) with + over Moore(1),
-    Remove(0.0), the elevation a second StencilArray; dest starts at zero.
-    Algorithmic bytes: water and elevation read, dest written (24 B/cell)."""
+    Remove(0.0) (and the same under Wrap and Reflect), the elevation a second
+    StencilArray; dest accumulates (it is read and written).  Algorithmic
+    bytes: water, elevation and dest read, dest written (32 B/cell)."""
     rng = np.random.Generator(np.random.PCG64(7))
     dev = torch.device("cuda", torch.cuda.current_device())
 
@@ -251,24 +252,30 @@ def bench_scatter(sh, torch, n: int = 8192, oracle_ok: bool = False):
 
     water, dem, dest = field(1.0), field(10.0), field(0.0)
     f = sh.Scatter.downhill_flow(0.1)
-    w = sh.StencilArray(water, sh.Moore(1), boundary=sh.Remove(0.0))
-    z = sh.StencilArray(dem, sh.Moore(1), boundary=sh.Remove(0.0))
 
-    def run():
-        sh.scatterstencil_(f, "+", dest, w, z, flags=sh._abi.ASYNC)
-    run()
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    reps = 10
-    e0.record()
-    for _ in range(reps):
+    def timed(bc, reps=10):
+        w = sh.StencilArray(water, sh.Moore(1), boundary=bc)
+        z = sh.StencilArray(dem, sh.Moore(1), boundary=bc)
+
+        def run():
+            sh.scatterstencil_(f, "+", dest, w, z, flags=sh._abi.ASYNC)
         run()
-    e1.record()
-    torch.cuda.synchronize()
-    t = e0.elapsed_time(e1) / 1e3 / reps
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            run()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3 / reps
+
     cells = n * n
+    t = timed(sh.Remove(0.0))
     out = {"workload": f"scatter {n}x{n} Float64: downhill flow 0.1, +, Moore(1), Remove(0.0), elevation field",
-           "kernel_us": t * 1e6, "cells_per_s": cells / t, "gbs": 24 * cells / t / 1e9}
+           "kernel_us": t * 1e6, "cells_per_s": cells / t, "gbs": 32 * cells / t / 1e9}
+    for name, bc in (("wrap", sh.Wrap()), ("reflect", sh.Reflect())):
+        tb = timed(bc)
+        out[f"{name}_kernel_us"] = tb * 1e6
     if oracle_ok:
         import oracle
         m = 1024  # a bounded sample: the reference's loop on one thread
@@ -781,7 +788,8 @@ def main():
                           "heat_2d_iterate": bench_heat_iterate_2d(sh, torch),
                           "served_heat_3d": bench_served(sh, torch),
                           "coupled_fields": bench_multi(sh, torch),
-                          "scatter_downhill": bench_scatter(sh, torch)}
+                          "scatter_downhill": bench_scatter(sh, torch),
+                          "runtime_offsets": bench_runtime(sh, torch)}
     print(json.dumps(out), flush=True)
 
 

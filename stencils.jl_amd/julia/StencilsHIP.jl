@@ -30,6 +30,7 @@ import AMDGPU
 using StaticArrays: SVector
 
 export LifeRule, Laplacian, Terms, Scaled, mapstencil_steps!, mapstencil_batch!, sth_available
+export ScatterConstant, ScatterCenter, ScatterScaled, Downhill, DownhillFlow
 export Server, submit!, start!, completions!, drain!, close!
 
 const libsth = get(ENV, "STENCILS_HIP_LIB",
@@ -103,6 +104,12 @@ struct SthTerm
     coef::Float64
     scaled::Int32
     reserved::Int32
+end
+
+struct SthScatterRule
+    kind::Int32
+    combine::Int32
+    coef::Float64
 end
 
 struct SthServerConfig
@@ -410,6 +417,112 @@ function Stencils.gatherstencil!(f::Terms, dest::StencilArray{<:Any,<:Any,<:Any,
     st == STH_ENOTSUP || _throw(st)
     return invoke(Stencils.gatherstencil!, Tuple{Any,Any,AbstractStencilArray,Vararg{AbstractArray}},
                   f, dest, source, arg1, args...)
+end
+
+# ------------------------------------------------------------ scatter rules
+"""
+    ScatterConstant(c), ScatterCenter(), ScatterScaled(c), Downhill(c), DownhillFlow(c)
+
+The `f(hood, args...) -> SVector` functions of `scatterstencil!`
+(src/scatterstencil.jl:1-47) that the device path covers: every neighbour
+receives `c`, the centre value, the centre value times `c`; and, with a
+second StencilArray `dem` of heights, `Δ = center(dem) - n` per neighbour
+height `n` and `Δ > 0 ? center(hood) * c : 0` (`Downhill`) or
+`Δ > 0 ? center(hood) * Δ * c : 0` (`DownhillFlow`, the reference's water
+example).  Their CPU methods are what the reference path runs.
+"""
+abstract type ScatterRule end
+struct ScatterConstant{C} <: ScatterRule
+    c::C
+end
+struct ScatterCenter <: ScatterRule end
+struct ScatterScaled{C} <: ScatterRule
+    c::C
+end
+struct Downhill{C} <: ScatterRule
+    c::C
+end
+struct DownhillFlow{C} <: ScatterRule
+    c::C
+end
+(s::ScatterConstant)(hood) = map(_ -> s.c, neighbors(hood))
+(s::ScatterCenter)(hood) = map(_ -> center(hood), neighbors(hood))
+(s::ScatterScaled)(hood) = map(_ -> center(hood) * s.c, neighbors(hood))
+function (s::Downhill)(hood, dem)
+    a, z = center(hood), center(dem)
+    map(n -> (Δ = z - n; Δ > 0 ? a * s.c : zero(a)), neighbors(dem))
+end
+function (s::DownhillFlow)(hood, dem)
+    a, z = center(hood), center(dem)
+    map(n -> (Δ = z - n; Δ > 0 ? a * Δ * s.c : zero(a)), neighbors(dem))
+end
+
+_rule_kind(::ScatterConstant) = Int32(0)
+_rule_kind(::ScatterCenter) = Int32(1)
+_rule_kind(::ScatterScaled) = Int32(2)
+_rule_kind(::Downhill) = Int32(3)
+_rule_kind(::DownhillFlow) = Int32(4)
+_rule_coef(::ScatterCenter, ::Type) = 0.0
+_rule_coef(r::ScatterRule, ::Type{T}) where T = r.c isa T ? Float64(r.c) : nothing  # else the values promote
+_combine(::typeof(+)) = Int32(0)
+_combine(::typeof(max)) = Int32(2)
+_combine(::typeof(min)) = Int32(3)
+_combine(_) = nothing
+
+# sth_scatter for a device source (and second array); STH_ENOTSUP when the
+# call is outside the family
+function _scatter(f::ScatterRule, op, dest::ROCArray{T,2}, source, args, flags::UInt32) where T
+    T <: Union{Float32,Float64} || return STH_ENOTSUP
+    comb = _combine(op)
+    coef = _rule_coef(f, T)
+    (comb === nothing || coef === nothing) && return STH_ENOTSUP
+    needs_aux = f isa Union{Downhill,DownhillFlow}
+    length(args) == (needs_aux ? 1 : 0) || return STH_ENOTSUP
+    keep = Any[]
+    ts = _term(sum, source, T, keep)
+    ts === nothing && return STH_ENOTSUP
+    ta = needs_aux ? _term(sum, args[1], T, keep) : nothing
+    needs_aux && (ta === nothing || !(args[1] isa AbstractStencilArray)) && return STH_ENOTSUP
+    rule = SthScatterRule(_rule_kind(f), comb, coef)
+    da = _array(dest, size(dest), 0)
+    return GC.@preserve keep ccall((:sth_scatter, libsth), Cint,
+        (Ptr{Cvoid}, Ref{SthTerm}, Ptr{SthTerm}, Ref{SthArray}, Ref{SthScatterRule}, UInt32, Ptr{Cvoid}),
+        _context(), ts, ta === nothing ? C_NULL : Ref(ta), da, rule, flags, _stream())
+end
+
+const DevSA2 = AbstractStencilArray{<:Any,<:Any,2,<:ROCArray}
+
+function Stencils.scatterstencil!(f::ScatterRule, op::OP, dest::ROCArray{<:Any,2}, source::DevSA2,
+                                  args::AbstractArray...) where OP
+    st = _scatter(f, op, dest, source, args, UInt32(0))
+    st == STH_OK && return dest
+    st == STH_ENOTSUP || _throw(st)
+    return invoke(Stencils.scatterstencil!, Tuple{Any,Any,AbstractArray,AbstractStencilArray,Vararg{AbstractArray}},
+                  f, op, dest, source, args...)
+end
+
+# the SwitchingStencilArray forms (src/scatterstencil.jl:117-136): dest is
+# cleared in the same call (STH_SCATTER_ZERO_DEST) and the array switched
+const STH_SCATTER_ZERO_DEST = UInt32(64)
+function Stencils.scatterstencil!(f::ScatterRule, op::OP,
+                                  A::SwitchingStencilArray{<:Any,<:Any,2,<:ROCArray},
+                                  src::DevSA2, args::AbstractArray...) where OP
+    st = _scatter(f, op, dest(A), src, args, STH_SCATTER_ZERO_DEST)
+    st == STH_OK && return switch(A)
+    st == STH_ENOTSUP || _throw(st)
+    return invoke(Stencils.scatterstencil!, Tuple{Any,Any,SwitchingStencilArray,AbstractStencilArray,
+                                                  Vararg{AbstractArray}}, f, op, A, src, args...)
+end
+function Stencils.scatterstencil!(f::ScatterRule, op::OP,
+                                  A::SwitchingStencilArray{<:Any,<:Any,2,<:ROCArray},
+                                  args::AbstractArray...) where OP
+    pd = padding(A) isa Halo ? Halo{:in}() : padding(A)
+    src = StencilArray(source(A), stencil(A), boundary(A), pd)
+    st = _scatter(f, op, dest(A), src, args, STH_SCATTER_ZERO_DEST)
+    st == STH_OK && return switch(A)
+    st == STH_ENOTSUP || _throw(st)
+    return invoke(Stencils.scatterstencil!, Tuple{Any,Any,SwitchingStencilArray,Vararg{AbstractArray}},
+                  f, op, A, args...)
 end
 
 """
