@@ -323,6 +323,67 @@ def bench_runtime(sh, torch, n: int = 8192):
     return out
 
 
+def bench_iterate_batch(sh, torch, n: int = 64, seed: int = 8):
+    """Many iterations with uneven step counts: n Float32 SwitchingStencilArrays
+    (sides log-uniform 64..1024, aspect 0.5..2, PCG64 seed 8) of 2-D heat,
+    Laplacian(0.2) over VonNeumann(1) under Wrap, each iterated U{10..200}
+    steps.  Three ways: the reference's loop (one mapstencil! per array and
+    step), one sth_iterate per array (its steps graph-replayed), and one
+    sth_iterate_batch call (all arrays advance together, one launch per round;
+    its schedule graph-replayed).  Wall time with synchronisation, inputs
+    resident; the graphs are captured in a warm-up call."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    dev = torch.device("cuda", torch.cuda.current_device())
+    f = sh.Laplacian(0.2)
+    arrays, steps, cells = [], [], 0
+    for _ in range(n):
+        side = float(np.exp(rng.uniform(np.log(64), np.log(1024))))
+        asp = rng.uniform(0.5, 2.0)
+        shape = (max(8, int(side * np.sqrt(asp))), max(8, int(side / np.sqrt(asp))))
+        t = sh.colmajor_empty(shape, torch.float32, dev)
+        t.copy_(torch.from_numpy(rng.random(shape, dtype=np.float32)).to(dev))
+        arrays.append(sh.SwitchingStencilArray(t, sh.VonNeumann(1), boundary=sh.Wrap()))
+        k = int(rng.integers(10, 201))
+        steps.append(k)
+        cells += shape[0] * shape[1] * k
+
+    def wall(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        best = float("inf")
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        return best
+
+    def loop_steps():
+        for i, A in enumerate(arrays):
+            for _ in range(steps[i]):
+                A = sh.mapstencil_(f, A, flags=sh._abi.ASYNC)
+            arrays[i] = A
+
+    def loop_iterate():
+        for i, A in enumerate(arrays):
+            arrays[i] = sh.mapstencil_steps(f, A, steps[i], graph=True, flags=sh._abi.ASYNC)
+
+    def batched():
+        res = sh.mapstencil_steps_batch([(f, A, k) for A, k in zip(arrays, steps)], graph=True,
+                                        flags=sh._abi.ASYNC)
+        arrays[:] = res
+
+    t_loop = wall(loop_steps, reps=1)
+    t_iter = wall(loop_iterate)
+    t_batch = wall(batched)
+    return {"workload": f"{n} uneven Float32 2-D heat arrays (sides 64..1024), Laplacian(0.2), VonNeumann(1), "
+                        "Wrap, U{10..200} steps each",
+            "cell_steps": cells, "max_steps": max(steps),
+            "reference_loop_ms": t_loop * 1e3, "per_array_iterate_ms": t_iter * 1e3,
+            "iterate_batch_ms": t_batch * 1e3, "cell_steps_per_s": cells / t_batch,
+            "speedup_vs_reference_loop": t_loop / t_batch, "speedup_vs_per_array_iterate": t_iter / t_batch}
+
+
 def gaussian_weights(sigma: float = 1.5, R: int = 3) -> np.ndarray:
     a = np.arange(-R, R + 1)
     W = np.exp(-(a[:, None] ** 2 + a[None, :] ** 2) / (2 * sigma ** 2))
@@ -674,7 +735,7 @@ def main():
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the secondary workloads (mean blur, Gaussian)")
     ap.add_argument("--only", choices=["life", "heat3d-step", "heat3d", "heat2d", "served", "batched", "gaussian",
-                                       "meanblur", "multi", "scatter", "runtime"],
+                                       "meanblur", "multi", "scatter", "runtime", "iterbatch"],
                     help="run one secondary workload alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -698,7 +759,8 @@ def main():
                    "meanblur": lambda: bench_mean_blur(sh, torch, False),
                    "multi": lambda: bench_multi(sh, torch),
                    "scatter": lambda: bench_scatter(sh, torch, oracle_ok=True),
-                   "runtime": lambda: bench_runtime(sh, torch)}[args.only]
+                   "runtime": lambda: bench_runtime(sh, torch),
+                   "iterbatch": lambda: bench_iterate_batch(sh, torch)}[args.only]
         print(json.dumps({args.only: run_one()}), flush=True)
         return
 
@@ -789,7 +851,8 @@ def main():
                           "served_heat_3d": bench_served(sh, torch),
                           "coupled_fields": bench_multi(sh, torch),
                           "scatter_downhill": bench_scatter(sh, torch),
-                          "runtime_offsets": bench_runtime(sh, torch)}
+                          "runtime_offsets": bench_runtime(sh, torch),
+                          "iterate_batch_uneven": bench_iterate_batch(sh, torch)}
     print(json.dumps(out), flush=True)
 
 

@@ -354,6 +354,25 @@ int sth_map_batch(sth_ctx* ctx, int32_t njobs, const sth_job* jobs, uint32_t fla
                   void* stream);
 
 /*
+ * Many independent iterations with uneven step counts in one call: job i is
+ * a SwitchingStencilArray whose buffers are jobs[i].src (its source) and
+ * jobs[i].dst, advanced by steps[i] calls of A = mapstencil!(f_i, A)
+ * (src/gatherstencil.jl:77-83, switch src/array.jl:610-611); the reference
+ * counterpart is a loop over the arrays, each looping over its own steps.
+ * Round r advances every array with more than r steps by one step, as one
+ * sth_map_batch launch set over the arrays still running (one launch per
+ * (element type, shape, reduction), large arrays on their own launches).
+ * With STH_GRAPH the whole schedule is captured once as a hipGraph -- its
+ * job tables in device memory the graph owns -- and replayed by later calls
+ * with the same arrays, functions and step counts.  current[i] receives
+ * which buffer holds array i's newest state (0: src, 1: dst).  Each result,
+ * halo included, equals sth_iterate on that array alone.  STH_BATCH_SERIAL
+ * runs one launch per array and step (the comparison path).
+ */
+int sth_iterate_batch(sth_ctx* ctx, int32_t njobs, const sth_job* jobs, const int64_t* steps,
+                      uint32_t flags, int32_t* current, void* stream);
+
+/*
  * sth_map_batch without allocating: the same calls, kernels and results as
  * sth_map_batch (each result identical to sth_map on that job alone), but
  * no device or pinned host memory is allocated, no host buffer is copied to

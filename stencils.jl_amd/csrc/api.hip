@@ -743,6 +743,11 @@ struct sth_ctx {
   hipStream_t capture = nullptr;  // private stream used only for graph capture
   std::mutex mu;
   std::map<std::string, hipGraphExec_t> graphs;
+  // device job tables a captured batched iteration's graph launches read
+  // (owned by that graph, freed with it)
+  std::map<std::string, std::vector<void*>> graph_tables;
+  std::vector<void*>* capture_tables = nullptr;  // set while such a graph is captured
+  hipStream_t upload = nullptr;                  // their uploads (non-blocking, outside the graph)
   // streams a batched call spreads its launches over (default priority),
   // created on first use, with the fork and join events that order them
   // after and before the caller's stream
@@ -972,6 +977,19 @@ int prepare_group(int64_t key, std::vector<BatchJob>& bj, const BatchKernelEntry
 // used slot is refilled once its last launch has finished.
 template <class F>
 int with_device_table(sth_ctx* ctx, const void* data, size_t bytes, hipStream_t st, F&& launch) {
+  if (ctx->capture_tables) {
+    // a graph is being captured: the table gets device memory of its own,
+    // filled now (before any replay), which the graph keeps until evicted
+    void* dev = nullptr;
+    STH_HIP(hipMalloc(&dev, std::max<size_t>(bytes, 256)));
+    ctx->capture_tables->push_back(dev);
+    if (!ctx->upload) STH_HIP(hipStreamCreateWithFlags(&ctx->upload, hipStreamNonBlocking));
+    STH_HIP(hipMemcpyAsync(dev, data, bytes, hipMemcpyHostToDevice, ctx->upload));
+    STH_HIP(hipStreamSynchronize(ctx->upload));
+    launch(static_cast<const void*>(dev));
+    STH_HIP(hipGetLastError());
+    return STH_OK;
+  }
   std::lock_guard<std::mutex> lk(ctx->mu);
   BatchSlot* hit = nullptr;
   for (BatchSlot& c : ctx->slots)
@@ -1137,6 +1155,18 @@ void load_tune_file(sth_ctx* ctx) {
 }
 }  // namespace
 
+// Drop every cached graph (and the job tables batched ones own) once the
+// device is idle: the cache's bound, and context teardown.  ctx->mu held.
+static int clear_graphs(sth_ctx* ctx) {
+  STH_HIP(hipDeviceSynchronize());
+  for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+  ctx->graphs.clear();
+  for (auto& kv : ctx->graph_tables)
+    for (void* d : kv.second) (void)hipFree(d);
+  ctx->graph_tables.clear();
+  return STH_OK;
+}
+
 extern "C" {
 
 int sth_version(void) { return STH_ABI_VERSION * 10000 + 0; }
@@ -1180,7 +1210,11 @@ int sth_ctx_destroy(sth_ctx* ctx) {
   STH_GUARD_BEGIN
   if (!ctx) return STH_OK;
   (void)hipSetDevice(ctx->device);
-  for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    (void)clear_graphs(ctx);
+  }
+  if (ctx->upload) (void)hipStreamDestroy(ctx->upload);
   for (auto& sl : ctx->slots) {
     if (sl.done) (void)hipEventSynchronize(sl.done);
     if (sl.done) (void)hipEventDestroy(sl.done);
@@ -1687,9 +1721,7 @@ int sth_iterate(sth_ctx* ctx, const sth_array* a, const sth_array* b, const sth_
       if (ctx->graphs.size() >= kMaxGraphs) {
         // bound the cache (a long-running caller iterating ever new buffers
         // would grow it without end): wait for the device, then drop it all
-        STH_HIP(hipDeviceSynchronize());
-        for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
-        ctx->graphs.clear();
+        if (int r3 = clear_graphs(ctx)) return r3;
       }
       ctx->graphs.emplace(key, x);
       for (int64_t i = 0; i < times; ++i) STH_HIP(hipGraphLaunch(x, s));
@@ -1748,6 +1780,113 @@ int sth_map_batch(sth_ctx* ctx, int32_t njobs, const sth_job* jobs, uint32_t fla
     ptrs[i] = &plans[i];
   }
   if ((rc = launch_batched(ctx, ptrs.data(), njobs, serial_only, s))) return rc;
+  if (!(flags & STH_ASYNC)) STH_HIP(hipStreamSynchronize(s));
+  return STH_OK;
+  STH_GUARD_END
+}
+
+int sth_iterate_batch(sth_ctx* ctx, int32_t njobs, const sth_job* jobs, const int64_t* steps, uint32_t flags,
+                      int32_t* current, void* stream) {
+  STH_GUARD_BEGIN
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  if (njobs < 0 || (njobs > 0 && (!jobs || !steps || !current))) return fail(STH_EINVAL, "bad job list");
+  auto s = static_cast<hipStream_t>(stream);
+  const bool serial_only = (flags & STH_BATCH_SERIAL) != 0;
+  // each array's two steps: source -> dest (even rounds), dest -> source
+  std::vector<Plan> fwd(static_cast<size_t>(njobs)), bwd(static_cast<size_t>(njobs));
+  std::vector<int32_t> order;
+  int64_t maxsteps = 0;
+  for (int32_t i = 0; i < njobs; ++i) {
+    const sth_job& jb = jobs[i];
+    if (steps[i] < 0) return fail(STH_EINVAL, "job %d: negative step count", i);
+    if (jb.src.dtype != jb.dst.dtype || jb.src.halo != jb.dst.halo)
+      return fail(STH_EINVAL, "job %d: source and dest buffers differ in type or halo", i);
+    if (jb.op.kind == STH_OP_MEAN && !is_float(jb.src.dtype))
+      return fail(STH_EINVAL, "job %d: mapstencil! cannot store a Float64 mean into an integer buffer", i);
+    std::vector<Variant> cf, cb;
+    std::string group;
+    if ((rc = make_plan(&jb.src, &jb.dst, &jb.stencil, jb.boundary, jb.padval, jb.padding, &jb.op,
+                        flags & ~STH_GENERIC, &fwd[i], &cf, &group)) ||
+        (rc = make_plan(&jb.dst, &jb.src, &jb.stencil, jb.boundary, jb.padval, jb.padding, &jb.op,
+                        flags & ~STH_GENERIC, &bwd[i], &cb))) {
+      std::string m = g_err;
+      return fail(rc, "job %d: %s", i, m.c_str());
+    }
+    int idx = 0;
+    if ((rc = tune_pick(ctx, fwd[i], cf, group, flags, s, false, true, &idx))) return rc;
+    apply_variant(fwd[i], cf[idx]);
+    apply_variant(bwd[i], cb[idx]);
+    current[i] = static_cast<int32_t>(steps[i] % 2);
+    if (steps[i] > 0) order.push_back(i);
+    maxsteps = std::max(maxsteps, steps[i]);
+  }
+  // round r advances every array with more than r steps: with the arrays
+  // ordered by step count (descending) that is a prefix of `order`, the
+  // same prefix until its shortest member is done
+  std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return steps[x] > steps[y]; });
+  auto enqueue = [&](hipStream_t st) -> int {
+    std::vector<const Plan*> even, odd;
+    size_t m = order.size();
+    for (int64_t r = 0; r < maxsteps;) {
+      while (m > 0 && steps[order[m - 1]] <= r) --m;
+      even.clear();
+      odd.clear();
+      for (size_t j = 0; j < m; ++j) {
+        even.push_back(&fwd[order[j]]);
+        odd.push_back(&bwd[order[j]]);
+      }
+      for (const int64_t r_end = steps[order[m - 1]]; r < r_end; ++r)
+        if (int r2 = launch_batched(ctx, (r % 2 ? odd : even).data(), static_cast<int32_t>(m), serial_only, st))
+          return r2;
+    }
+    return STH_OK;
+  };
+  if (!(flags & STH_GRAPH) || maxsteps == 0) {
+    if ((rc = enqueue(s))) return rc;
+  } else {
+    // the whole schedule as one graph, captured once per (arrays, functions,
+    // step counts) and replayed; its job tables are device memory it owns
+    std::string key("B", 1);
+    key.push_back(serial_only ? 's' : 'b');
+    for (int32_t i : order) {
+      key.append(reinterpret_cast<const char*>(&fwd[i]), sizeof(Plan));
+      key.append(reinterpret_cast<const char*>(&bwd[i]), sizeof(Plan));
+      key.append(reinterpret_cast<const char*>(&steps[i]), sizeof(int64_t));
+    }
+    bool done = false;
+    {
+      std::lock_guard<std::mutex> lk(ctx->mu);
+      auto it = ctx->graphs.find(key);
+      if (it != ctx->graphs.end()) {
+        STH_HIP(hipGraphLaunch(it->second, s));
+        done = true;
+      }
+    }
+    if (!done) {
+      std::vector<void*> tables;
+      ctx->capture_tables = &tables;
+      hipError_t e = hipStreamBeginCapture(ctx->capture, hipStreamCaptureModeRelaxed);
+      const int r2 = e == hipSuccess ? enqueue(ctx->capture) : STH_OK;
+      hipGraph_t g = nullptr;
+      if (e == hipSuccess) e = hipStreamEndCapture(ctx->capture, &g);
+      ctx->capture_tables = nullptr;
+      hipGraphExec_t x = nullptr;
+      if (r2 == STH_OK && e == hipSuccess) e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+      if (g) (void)hipGraphDestroy(g);
+      if (r2 || e != hipSuccess) {
+        for (void* d : tables) (void)hipFree(d);
+        if (r2) return r2;
+        return fail(STH_EDEVICE, "batched iteration graph: %s", hipGetErrorString(e));
+      }
+      std::lock_guard<std::mutex> lk(ctx->mu);
+      if (ctx->graphs.size() >= kMaxGraphs && (rc = clear_graphs(ctx))) return rc;
+      ctx->graphs.emplace(key, x);
+      ctx->graph_tables.emplace(key, std::move(tables));
+      STH_HIP(hipGraphLaunch(x, s));
+    }
+  }
+  STH_HIP(hipGetLastError());
   if (!(flags & STH_ASYNC)) STH_HIP(hipStreamSynchronize(s));
   return STH_OK;
   STH_GUARD_END

@@ -29,7 +29,7 @@ using AMDGPU: ROCArray
 import AMDGPU
 using StaticArrays: SVector
 
-export LifeRule, Laplacian, Terms, Scaled, mapstencil_steps!, mapstencil_batch!, sth_available
+export LifeRule, Laplacian, Terms, Scaled, mapstencil_steps!, mapstencil_batch!, mapstencil_steps_batch!, sth_available
 export ScatterConstant, ScatterCenter, ScatterScaled, Downhill, DownhillFlow
 export Server, submit!, start!, completions!, drain!, close!
 
@@ -608,6 +608,52 @@ function mapstencil_batch!(calls::AbstractVector)
         _throw(st)
     end
     return calls
+end
+
+"""
+    mapstencil_steps_batch!(items; graph=true)
+
+Many iterations with their own step counts in one call (`sth_iterate_batch`):
+`items` is a vector of `(f, A::SwitchingStencilArray, steps)` over device
+arrays; every array advances one step per round together with the others
+still running, and with `graph` the whole schedule is captured once and
+replayed on later calls with the same arrays and counts.  Returns the
+switched arrays, each as its own loop `for _ in 1:steps; A = mapstencil!(f,
+A); end` (src/gatherstencil.jl:77-83) would; calls outside the accelerated
+family run that loop.
+"""
+function mapstencil_steps_batch!(items::AbstractVector; graph::Bool=true)
+    keep = Any[]
+    jobs = SthJob[]
+    steps = Int64[]
+    fallback() = map(items) do (f, A, k)
+        for _ in 1:k
+            A = mapstencil!(f, A)
+        end
+        A
+    end
+    for (f, A, k) in items
+        T = eltype(A)
+        op = haskey(DTYPES, T) ? _op(f, T) : nothing
+        pv = _padval(boundary(A), T)
+        (op === nothing || pv === nothing) && return fallback()
+        offs, kind, r, n, l = _stencil_desc(stencil(A))
+        w = _weights(stencil(A), T)
+        push!(keep, offs, w, pv)
+        sd = SthStencil(kind, r, n, l, pointer(offs),
+                        w === nothing ? C_NULL : Ptr{Cvoid}(pointer(w)),
+                        w === nothing ? Int32(-1) : DTYPES[eltype(w)], 0)
+        push!(jobs, SthJob(_array(source(A), size(A), _halo(A)), _array(dest(A), size(A), _halo(A)),
+                           sd, op, Base.unsafe_convert(Ptr{Cvoid}, pv), _bc(boundary(A)), _pad(padding(A))))
+        push!(steps, Int64(k))
+    end
+    cur = zeros(Int32, length(jobs))
+    st = GC.@preserve keep jobs ccall((:sth_iterate_batch, libsth), Cint,
+        (Ptr{Cvoid}, Int32, Ptr{SthJob}, Ptr{Int64}, UInt32, Ptr{Int32}, Ptr{Cvoid}),
+        _context(), Int32(length(jobs)), jobs, steps, graph ? STH_GRAPH : UInt32(0), cur, _stream())
+    st == STH_ENOTSUP && return fallback()
+    st == STH_OK || _throw(st)
+    return [c == 1 ? switch(A) : A for ((_, A, _), c) in zip(items, cur)]
 end
 
 # ------------------------------------------------------------ serving loop

@@ -10,7 +10,7 @@ from .array import (StencilArray, SwitchingStencilArray, colmajor_empty, dest,
                     parent, source, switch)
 from .boundary import Conditional, Halo, Reflect, Remove, Use, Wrap, padval
 from .gather import (BatchPlan, autotune, gatherstencil, gatherstencil_, mapstencil, mapstencil_,
-                     mapstencil_batch, mapstencil_steps, result_dtype)
+                     mapstencil_batch, mapstencil_steps, mapstencil_steps_batch, result_dtype)
 from .scatter import Scatter, scatterstencil_
 from .server import Completion, Server, serve_latency_summary
 from .ops import (Laplacian, LifeRule, Scaled, Terms, Unsupported, center, integer_fold_wraps,

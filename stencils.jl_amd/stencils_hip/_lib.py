@@ -48,6 +48,9 @@ EXPORTS = {
                                      ctypes.c_uint32, ctypes.c_void_p]),
     "sth_map_batch_static": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(_abi.SthJob),
                                             ctypes.c_uint32, ctypes.c_void_p]),
+    "sth_iterate_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(_abi.SthJob),
+                                         ctypes.POINTER(ctypes.c_int64), ctypes.c_uint32,
+                                         ctypes.POINTER(ctypes.c_int32), ctypes.c_void_p]),
     "sth_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_abi.SthArray),
                                     ctypes.POINTER(_abi.SthArray), ctypes.POINTER(_abi.SthStencil),
                                     ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32,

@@ -567,6 +567,56 @@ def mapstencil_batch(calls, serial: bool = False, flags: int = 0):
     BatchPlan(calls).run(serial=serial, flags=flags)
 
 
+def mapstencil_steps_batch(items, graph: bool = True, serial: bool = False, flags: int = 0):
+    """Many iterations with their own step counts in one library call
+    (sth_iterate_batch): `items` is a sequence of (f, A, steps), A a
+    device-resident SwitchingStencilArray; each A gets `steps` calls of
+    mapstencil!(f, A), all arrays advancing together one step per round (one
+    launch per element type, shape and reduction over the arrays still
+    running).  With `graph` the whole schedule replays as a captured hipGraph
+    on later calls with the same arrays, functions and step counts.  Returns
+    the arrays switched as each one's own loop would return it; each equals
+    mapstencil_steps(f, A, steps) on that array alone."""
+    _require_backend()
+    items = list(items)
+    n = len(items)
+    jobs = (_abi.SthJob * max(n, 1))()
+    steps = (ctypes.c_int64 * max(n, 1))()
+    cur = (ctypes.c_int32 * max(n, 1))()
+    keep = _Call()
+    dev = None
+    for i, (f, A, k) in enumerate(items):
+        red = resolve(f)
+        if not isinstance(A, SwitchingStencilArray) or not is_device(A.source):
+            raise Unsupported("batched iterations take device-resident SwitchingStencilArrays")
+        if result_dtype(red, A) != A.sth_dtype():
+            raise ValueError("the result type differs from the array's element type")
+        if int(k) < 0:
+            raise ValueError("negative step count")
+        dt = A.sth_dtype()
+        j = jobs[i]
+        j.src = _desc(A.source, A.size(), A.halo())
+        j.dst = _desc(A.dest, A.size(), A.halo())
+        j.stencil = stencil_descriptor(A.stencil, dt, keep)
+        j.op = red.op(_op_flags(dt))
+        pv = _padval_bytes(A)
+        if pv is not None:
+            j.padval = ctypes.cast(keep.hold(ctypes.create_string_buffer(pv, len(pv))), ctypes.c_void_p)
+        j.boundary = A.boundary.code
+        j.padding = A.padding.code
+        steps[i] = int(k)
+        d = _device_index(A.source)
+        if dev is None:
+            dev = d
+        elif d != dev:
+            raise ValueError("all arrays of a batch must be on one device")
+    if n == 0:
+        return []
+    fl = flags | (_abi.GRAPH if graph else 0) | (_abi.BATCH_SERIAL if serial else 0)
+    _lib.check(_lib.lib().sth_iterate_batch(_lib.context(dev), n, jobs, steps, fl, cur, _raw_stream(dev)))
+    return [A.switch() if cur[i] == 1 else A for i, (_, A, _k) in enumerate(items)]
+
+
 # Deprecated aliases of the reference (src/gatherstencil.jl:127-128).
 mapstencil = gatherstencil
 mapstencil_ = gatherstencil_
