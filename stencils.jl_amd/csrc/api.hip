@@ -534,6 +534,7 @@ int make_plan(const sth_array* src, const sth_array* dst, const sth_stencil* st,
         for (int i = 0; i < 3 * L; ++i) o8[i] = static_cast<int8_t>(offs3[i]);
         std::string why;
         const JitKernel* jk = jit_stream_kernel(sdt, op->kind, L, o8.data(), reach[0], reach[1], reach[2], v, ky, &why);
+        if (!jk) jit_note("stream kernel", why);
         if (jk) {
           Variant jv;
           jv.jit = jk->fn;
@@ -1443,7 +1444,87 @@ int sth_map_multi(sth_ctx* ctx, int32_t nterms, const sth_term* terms, const sth
       pl.halo_fn(pl.hp, s);
       STH_HIP(hipGetLastError());
     }
-  if (n0 * n1 * n2 > 0) {
+  // Large calls: one runtime-compiled streaming pass per stencil argument,
+  // each folding its term into the running value (read back from dest after
+  // the first pass) together with the centre terms up to the next stencil
+  // argument.  Any pass that cannot be built leaves the call on the tiled
+  // kernel below.
+  bool done = false;
+  if (!(flags & STH_GENERIC) && dst->ndim >= 2 && jit_wanted(n0 * n1 * n2) && n0 * n1 * n2 > 0) {
+    std::vector<int32_t> folds;
+    for (int32_t t = 0; t < nterms; ++t)
+      if (!tbl[t].direct) folds.push_back(t);
+    bool ok = !folds.empty();
+    for (int32_t t : folds)
+      ok = ok && std::max(tbl[t].rx, std::max(tbl[t].ry, tbl[t].rz)) <= kJitMaxReach && tbl[t].opk != STH_OP_LIFE;
+    if (!ok) jit_note("multi-array call", "no stencil argument, a reach above 3 or the Life rule");
+    struct Pass {
+      const JitKernel* k;
+      MapParams p;
+      MultiEpiParams e;
+    };
+    std::vector<Pass> passes;
+    const int es = dtype_size(dt);
+    const int v = es == 1 ? 8 : 16 / es;
+    for (size_t f = 0; ok && f < folds.size(); ++f) {
+      const int32_t t = folds[f];
+      const int32_t lo = f == 0 ? 0 : t, hi = f + 1 < folds.size() ? folds[f + 1] : nterms;
+      const MultiTerm& q = tbl[t];
+      const int rmax = std::max(q.rx, std::max(q.ry, q.rz));
+      const int ky = (q.rz > 0 || rmax <= 2) ? 4 : 2;
+      Pass ps;
+      std::memset(static_cast<void*>(&ps.e), 0, sizeof ps.e);
+      std::string items;
+      int nc = 0;
+      for (int32_t u = lo; u < hi; ++u) {
+        if (u == t) {
+          items.push_back(tbl[u].scaled ? 'F' : 'f');
+          ps.e.coef[0] = tbl[u].coef;
+        } else {
+          items.push_back(tbl[u].scaled ? 'C' : 'c');
+          ps.e.base[nc] = tbl[u].src.base;
+          ps.e.s1[nc] = tbl[u].src.s1;
+          ps.e.s2[nc] = tbl[u].src.s2;
+          ps.e.coef[1 + nc] = tbl[u].coef;
+          ++nc;
+        }
+      }
+      ps.e.acc = mp.dst.base;
+      ps.e.acc_s1 = mp.dst.s1;
+      ps.e.acc_s2 = mp.dst.s2;
+      std::vector<int8_t> o8(3 * static_cast<size_t>(q.L));
+      for (int k = 0; k < q.L; ++k) {
+        o8[3 * k] = static_cast<int8_t>(q.offs[k] & 0xff);
+        o8[3 * k + 1] = static_cast<int8_t>((q.offs[k] >> 8) & 0xff);
+        o8[3 * k + 2] = static_cast<int8_t>((q.offs[k] >> 16) & 0xff);
+      }
+      std::string why;
+      ps.k = jit_multi_pass(dt, q.opk, q.L, o8.data(), q.rx, q.ry, q.rz, v, ky, f > 0, items, combine == STH_COMBINE_MUL,
+                            &why);
+      if (!ps.k) {
+        jit_note("multi-array pass", why);
+        ok = false;
+        break;
+      }
+      ps.p = plans[t].p;
+      ps.p.lpr_shift = lpr_shift_for(n0, v);
+      const int64_t lpr = int64_t(1) << ps.p.lpr_shift, rpw = 64 / lpr;
+      const int64_t jtx = (n0 + lpr * v - 1) / (lpr * v), jty = (n1 + 4 * ky * rpw - 1) / (4 * ky * rpw);
+      if (jtx * jty * n2 > (int64_t(1) << 31) - 1 || n2 > (int64_t(1) << 30)) {
+        ok = false;
+        break;
+      }
+      ps.p.tiles_x = static_cast<int32_t>(jtx);
+      ps.p.tiles_y = static_cast<int32_t>(jty);
+      ps.p.tiles_z = static_cast<int32_t>(n2);
+      passes.push_back(ps);
+    }
+    if (ok) {
+      for (const Pass& ps : passes) STH_HIP(jit_launch_multi(ps.k->fn, ps.p, ps.e, s));
+      done = true;
+    }
+  }
+  if (!done && n0 * n1 * n2 > 0) {
     rc = with_device_table(ctx, tbl.data(), tbl.size() * sizeof(MultiTerm), s, [&](const void* dev) {
       ke->fn(mp, static_cast<const MultiTerm*>(dev), lds, s);
     });

@@ -93,8 +93,17 @@ __device__ __forceinline__ int64_t resolve_row(int64_t y, int64_t n, int64_t h, 
   return map_index(yc, n, bc);
 }
 
-template <typename S, int OPK, class Sh, int V, int KY, class P>
-__device__ __forceinline__ void stream_body(const P& p, int bx, int by, int bz, int lane, int wave) {
+// What happens to a lane's V results of one row before they are stored:
+// nothing (a plain gatherstencil!), or the combination with further terms
+// of a multi-array gather (runtime-compiled kernels define their own).
+struct NoEpilogue {
+  template <typename R, int V>
+  __device__ __forceinline__ void apply(RawVec<R, V>&, int64_t, int64_t, int64_t, int64_t) const {}
+};
+
+template <typename S, int OPK, class Sh, int V, int KY, class P, class Epi = NoEpilogue>
+__device__ __forceinline__ void stream_body(const P& p, int bx, int by, int bz, int lane, int wave,
+                                            const Epi& epi = Epi{}) {
   using Raw = typename Elem<S>::raw;
   using Val = typename Staged<S>::type;
   using A = typename AccOf<S, OPK>::type;
@@ -240,6 +249,7 @@ __device__ __forceinline__ void stream_body(const P& p, int bx, int by, int bz, 
     }
     const int64_t y = y0 + j;
     if (y < n1) {
+      epi.apply(out, x0, y, z0, n0);
       ORaw* drow = dbase + y * p.dst.s1;
       if (x0 + V <= n0) {
         if (dst_aligned)
@@ -265,6 +275,20 @@ __global__ __launch_bounds__(256) void stream_kernel(MapParams p) {
   const int bz = byz / p.tiles_y;
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
   stream_body<S, OPK, Sh, V, KY>(p, bx, by, bz, static_cast<int>(threadIdx.x) & 63, wave);
+}
+
+// The streaming kernel with an epilogue (Epi::Params is its second
+// argument); instantiated only by runtime-compiled specialisations.
+template <typename S, int OPK, class Sh, int V, int KY, class Epi>
+__global__ __launch_bounds__(256) void stream_kernel_epi(MapParams p, typename Epi::Params e) {
+  const int nblocks = p.tiles_x * p.tiles_y * p.tiles_z;
+  const int bid = xcd_block_id(static_cast<int>(blockIdx.x), nblocks, p.xcd_remap);
+  const int bx = bid % p.tiles_x;
+  const int byz = bid / p.tiles_x;
+  const int by = byz % p.tiles_y;
+  const int bz = byz / p.tiles_y;
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
+  stream_body<S, OPK, Sh, V, KY>(p, bx, by, bz, static_cast<int>(threadIdx.x) & 63, wave, Epi{e});
 }
 
 // Natural vector width: 16 bytes per lane.

@@ -68,10 +68,10 @@ STENCILS = {1: [sh.VonNeumann(1, 1), sh.Positional((-1,), (0,), (2,)), sh.Moore(
 PADS = [sh.Conditional(), sh.Halo("out"), sh.Halo("in")]
 
 
-def _random_case(rng, dt, nd):
+def _random_case(rng, dt, nd, shapes=SHAPES):
     """(Terms, [numpy parents or plain arrays], [per-argument (stencil,
     boundary, padding) or None]) for one random multi-array call."""
-    shape = SHAPES[nd][int(rng.integers(len(SHAPES[nd])))]
+    shape = shapes[nd][int(rng.integers(len(shapes[nd])))]
     floats = np.dtype(dt).kind == "f"
     k = int(rng.integers(2, 5))
     terms, specs, arrays = [], [], []
@@ -150,6 +150,46 @@ def test_random_multi_against_oracle(gpu, oracle, int_wrap, dt, nd, seed):
     for ga, oa in zip(gargs, oargs):
         if isinstance(ga, sh.StencilArray):
             np.testing.assert_array_equal(_host(ga.parent()), oa.parent())
+
+
+LARGE = {2: [(1024, 1024), (1100, 960)], 3: [(128, 128, 64)]}
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("nd", [2, 3])
+@pytest.mark.parametrize("dt", [np.float32, np.float64, np.uint8, np.int32], ids=lambda d: np.dtype(d).name)
+def test_random_multi_large_against_oracle(gpu, oracle, int_wrap, dt, nd, seed):
+    """Calls of 2^20 cells and more run one runtime-compiled streaming pass
+    per stencil argument (the running value read back from dest between
+    passes, centre terms folded in on the way): random Terms as above, the
+    result and every Halo parent equal to the oracle's."""
+    rng = np.random.default_rng(5000 + 1000 * nd + 10 * seed + np.dtype(dt).itemsize)
+    f, arrays, specs = _random_case(rng, dt, nd, LARGE)
+    gargs = _args(arrays, specs, device=True)
+    oargs = _args(arrays, specs, device=False)
+    g = sh.mapstencil(f, *gargs)
+    o = oracle.mapstencil_multi(f, tuple(oargs))
+    np.testing.assert_array_equal(_host(g), o, err_msg=f"{f} {[s[:2] if s else None for s in specs]}")
+    for ga, oa in zip(gargs, oargs):
+        if isinstance(ga, sh.StencilArray):
+            np.testing.assert_array_equal(_host(ga.parent()), oa.parent())
+
+
+def test_large_leading_centre_terms(gpu, oracle):
+    """Centre terms before the first stencil argument and between two:
+    center(A) * 0.5 + sum(B) + center(C) + maximum(D) at 1024^2, Float64."""
+    rng = np.random.default_rng(3)
+    shape = (1024, 1024)
+    arrs = [np.asfortranarray(rng.random(shape)) for _ in range(4)]
+    specs = [None, (sh.Moore(1), sh.Wrap(), sh.Conditional()), None, (sh.Circle(2), sh.Reflect(), sh.Halo("out"))]
+    f = sh.Terms(sh.Scaled(0.5, sh.center), sh.sum, sh.center, sh.maximum)
+    # the first argument is the source, a StencilArray (read at its centre here)
+    specs[0] = (sh.Moore(1), sh.Wrap(), sh.Conditional())
+    gargs = _args(arrs, specs, device=True)
+    oargs = _args(arrs, specs, device=False)
+    g = sh.mapstencil(f, *gargs)
+    o = oracle.mapstencil_multi(f, tuple(oargs))
+    np.testing.assert_array_equal(_host(g), o)
 
 
 @pytest.mark.parametrize("pad", [sh.Conditional(), sh.Halo("out")], ids=repr)
