@@ -373,6 +373,17 @@ int sth_iterate_batch(sth_ctx* ctx, int32_t njobs, const sth_job* jobs, const in
                       uint32_t flags, int32_t* current, void* stream);
 
 /*
+ * Compile (with hipRTC, no device needed) a representative set of the
+ * runtime specialisations the library builds for calls outside its
+ * build-time catalogue -- streaming kernels over each storage type and
+ * multi-array passes -- and return how many failed, their compiler messages
+ * in `log` (NUL-terminated, at most `cap` bytes).  No counterpart in the
+ * reference; for installers and tests that check the runtime compiler works
+ * on a machine before any call depends on it.
+ */
+int sth_jit_selftest(char* log, size_t cap);
+
+/*
  * sth_map_batch without allocating: the same calls, kernels and results as
  * sth_map_batch (each result identical to sth_map on that job alone), but
  * no device or pinned host memory is allocated, no host buffer is copied to

@@ -1973,6 +1973,52 @@ int sth_iterate_batch(sth_ctx* ctx, int32_t njobs, const sth_job* jobs, const in
   STH_GUARD_END
 }
 
+int sth_jit_selftest(char* log, size_t cap) {
+  STH_GUARD_BEGIN
+  // representative specialisations of every generator: streaming kernels
+  // over each storage type, multi-array passes over each of their types and
+  // combine forms; compiled only (no device is needed or touched)
+  static const int8_t shape[6][3] = {{-2, 0, 0}, {0, -1, 0}, {0, 0, 0}, {1, 1, 0}, {2, -2, 0}, {-1, 2, 0}};
+  std::string out;
+  int failures = 0;
+  auto one = [&](const std::string& src, const std::string& expr, const std::string& what) {
+    std::string code, lowered, lg;
+    if (!jit_compile(src, expr, &code, &lowered, &lg)) {
+      ++failures;
+      out += what + ": " + lg.substr(0, 600) + "\n";
+    }
+  };
+  const int dts[] = {STH_BOOL, STH_U8, STH_I16, STH_I32, STH_I64, STH_F32, STH_F64, STH_BF16};
+  for (int dt : dts) {
+    const std::string targs = std::string(jit_type_name(dt)) + ", " + std::to_string(STH_OP_MAX) +
+                              ", sth::jit::Shape, 4, 4";
+    one("#include \"stream_kernel.hpp\"\n" + jit_shape_source(6, &shape[0][0], 2, 2, 0) +
+            "namespace sth {\ntemplate __global__ void stream_kernel<" + targs + ">(MapParams);\n}\n",
+        "&sth::stream_kernel<" + targs + ">", "stream " + std::string(jit_type_name(dt)));
+  }
+  struct M {
+    int dt, op;
+    const char* items;
+    bool acc, mul;
+  };
+  const M ms[] = {{STH_F32, STH_OP_LAPLACIAN, "fC", false, false}, {STH_F64, STH_OP_SUM, "Ccfc", false, true},
+                  {STH_U8, STH_OP_MAX, "cf", true, true},           {STH_I32, STH_OP_SUM, "fc", false, false},
+                  {STH_I64, STH_OP_MIN, "f", true, false}};
+  for (const M& m : ms) {
+    std::string src, expr;
+    const int v = m.dt == STH_U8 ? 8 : 16 / dtype_size(m.dt);
+    jit_multi_source(m.dt, m.op, 6, &shape[0][0], 2, 2, 0, v, 4, m.acc, m.items, m.mul, &src, &expr);
+    one(src, expr, std::string("multi pass ") + jit_type_name(m.dt) + " " + m.items);
+  }
+  if (log && cap) {
+    const size_t n = std::min(cap - 1, out.size());
+    std::memcpy(log, out.data(), n);
+    log[n] = '\0';
+  }
+  return failures;
+  STH_GUARD_END
+}
+
 int sth_map_batch_static(sth_ctx* ctx, int32_t njobs, const sth_job* jobs, uint32_t flags, void* stream) {
   STH_GUARD_BEGIN
   int rc = set_device(ctx);

@@ -48,6 +48,7 @@ EXPORTS = {
                                      ctypes.c_uint32, ctypes.c_void_p]),
     "sth_map_batch_static": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(_abi.SthJob),
                                             ctypes.c_uint32, ctypes.c_void_p]),
+    "sth_jit_selftest": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
     "sth_iterate_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(_abi.SthJob),
                                          ctypes.POINTER(ctypes.c_int64), ctypes.c_uint32,
                                          ctypes.POINTER(ctypes.c_int32), ctypes.c_void_p]),
