@@ -145,6 +145,8 @@ struct Plan {
   int32_t bklass;    // which batched kernel (one per (shape, op) variant)
   int32_t dtype;     // sth_dtype of the source
   LaunchFn fn_fused;      // several steps per launch (steady steps), or null
+  hipFunction_t jit_fused; // a runtime-compiled two-step kernel instead of fn_fused
+  int32_t jit_fused_3d;    // its grid: tiles_x * tiles_y (2-D) or * tiles_z (3-D)
   int32_t fused_gens;      // steps fn_fused advances per launch
   int32_t fused_tiles_x, fused_tiles_y, fused_tiles_z;  // its grid
   HaloParams hp;
@@ -603,6 +605,24 @@ int make_plan(const sth_array* src, const sth_array* dst, const sth_stencil* st,
       zmarch2_grid(n[0], n[1], n[2], e.rows, zm_waves, &pl.fused_tiles_x, &pl.fused_tiles_y, &pl.fused_tiles_z);
       break;
     }
+    // other radius-1 shapes and element types: the same kernel, runtime-compiled
+    if (!pl.fn_fused && sdt == odt && dtype_size(sdt) >= 2 && reach[0] <= 1 && reach[1] <= 1 && reach[2] <= 1 &&
+        op->kind != STH_OP_LIFE && jit_wanted(n[0] * n[1] * n[2])) {
+      std::vector<int8_t> o8(3 * static_cast<size_t>(L));
+      for (size_t i = 0; i < o8.size(); ++i) o8[i] = static_cast<int8_t>(offs3[i]);
+      std::string why;
+      if (const JitKernel* jk = jit_fused_kernel(sdt, op->kind, L, o8.data(), 1, 1, 1, 3, bc, &why)) {
+        const char* v = std::getenv("STH_ZM2_WAVES");
+        const int64_t zw = v ? std::max<int64_t>(1, std::atoll(v)) : int64_t(8192);
+        pl.jit_fused = jk->fn;
+        pl.jit_fused_3d = 1;
+        pl.fused_gens = 2;
+        zmarch2_grid(n[0], n[1], n[2], jit_zmarch2_rows(sdt), zw, &pl.fused_tiles_x, &pl.fused_tiles_y,
+                     &pl.fused_tiles_z);
+      } else {
+        jit_note("two-step 3-D kernel", why);
+      }
+    }
   }
 
   // two steps per launch for 2-D radius-1 catalogue stencils (Wrap / Remove /
@@ -633,6 +653,23 @@ int make_plan(const sth_array* src, const sth_array* dst, const sth_stencil* st,
       pl.fused_gens = 2;
       ymarch2_grid(n[0], n[1], e.rows, ym_waves, &pl.fused_tiles_x, &pl.fused_tiles_y, &pl.fused_tiles_z);
       break;
+    }
+    if (!pl.fn_fused && dtype_size(sdt) >= 2 && reach[0] <= 1 && reach[1] <= 1 && op->kind != STH_OP_LIFE &&
+        jit_wanted(n[0] * n[1])) {
+      std::vector<int8_t> o8(3 * static_cast<size_t>(L));
+      for (size_t i = 0; i < o8.size(); ++i) o8[i] = static_cast<int8_t>(offs3[i]);
+      std::string why;
+      if (const JitKernel* jk = jit_fused_kernel(sdt, op->kind, L, o8.data(), 1, 1, 0, 2, bc, &why)) {
+        const char* v = std::getenv("STH_YM2_WAVES");
+        const int64_t yw = v ? std::max<int64_t>(1, std::atoll(v)) : int64_t(8192);
+        pl.jit_fused = jk->fn;
+        pl.jit_fused_3d = 0;
+        pl.fused_gens = 2;
+        ymarch2_grid(n[0], n[1], 16 / dtype_size(sdt), yw, &pl.fused_tiles_x, &pl.fused_tiles_y,
+                     &pl.fused_tiles_z);
+      } else {
+        jit_note("two-step 2-D kernel", why);
+      }
     }
   }
 
@@ -700,7 +737,13 @@ int run_fused(const Plan& pl, hipStream_t s) {
   q.tiles_z = pl.fused_tiles_z;
   q.read_halo = 0;
   q.fuse_halo = 0;
-  pl.fn_fused(q, s);
+  if (pl.jit_fused) {
+    const unsigned nb = static_cast<unsigned>(q.tiles_x) * q.tiles_y * (pl.jit_fused_3d ? q.tiles_z : 1);
+    void* args[] = {&q};
+    STH_HIP(hipModuleLaunchKernel(pl.jit_fused, nb, 1, 1, 256, 1, 1, 0, s, args, nullptr));
+  } else {
+    pl.fn_fused(q, s);
+  }
   STH_HIP(hipGetLastError());
   return STH_OK;
 }
@@ -1757,7 +1800,8 @@ int sth_iterate(sth_ctx* ctx, const sth_array* a, const sth_array* b, const sth_
   // single steps, which leave both buffers and their halos as the
   // step-by-step reference does.
   const int64_t G = ab.fused_gens;
-  const bool fuse = !(flags & STH_NO_FUSE) && ab.fn_fused && ba.fn_fused && G >= 2 && rest >= 2 * G + 2;
+  const bool fuse = !(flags & STH_NO_FUSE) && (ab.fn_fused || ab.jit_fused) && (ba.fn_fused || ba.jit_fused) &&
+                    G >= 2 && rest >= 2 * G + 2;
   const int64_t units = fuse ? (rest - 2) / (2 * G) : 0;  // fused pairs
   const int64_t tail = rest - 2 * G * units;              // single steps (2 .. 2G + 1 when fused)
   auto enqueue = [&](hipStream_t st, bool first, int64_t nunits, int64_t nplain) -> int {
@@ -2009,6 +2053,16 @@ int sth_jit_selftest(char* log, size_t cap) {
     const int v = m.dt == STH_U8 ? 8 : 16 / dtype_size(m.dt);
     jit_multi_source(m.dt, m.op, 6, &shape[0][0], 2, 2, 0, v, 4, m.acc, m.items, m.mul, &src, &expr);
     one(src, expr, std::string("multi pass ") + jit_type_name(m.dt) + " " + m.items);
+  }
+  static const int8_t vn2[4][3] = {{0, -1, 0}, {-1, 0, 0}, {1, 0, 0}, {0, 1, 0}};
+  static const int8_t vn3[6][3] = {{0, 0, -1}, {0, -1, 0}, {-1, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  const int fdt[][3] = {{STH_I32, STH_OP_MAX, 2}, {STH_I64, STH_OP_SUM, 2}, {STH_F32, STH_OP_SUM, 2},
+                        {STH_I16, STH_OP_MIN, 3}, {STH_I64, STH_OP_SUM, 3}, {STH_F64, STH_OP_MAX, 3}};
+  for (const auto& c : fdt) {
+    std::string src, expr;
+    jit_fused_source(c[0], c[1], c[2] == 3 ? 6 : 4, c[2] == 3 ? &vn3[0][0] : &vn2[0][0], 1, 1, c[2] == 3 ? 1 : 0, c[2],
+                     STH_WRAP, &src, &expr);
+    one(src, expr, std::string("two-step ") + std::to_string(c[2]) + "-D " + jit_type_name(c[0]));
   }
   if (log && cap) {
     const size_t n = std::min(cap - 1, out.size());

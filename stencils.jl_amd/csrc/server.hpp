@@ -280,7 +280,8 @@ struct sth_server {
     return v;
   }
   bool solo(const Req& q) const {
-    return q.plan[0].fn_fused && q.plan[1].fn_fused && q.cells >= solo_cells();
+    return (q.plan[0].fn_fused || q.plan[0].jit_fused) && (q.plan[1].fn_fused || q.plan[1].jit_fused) &&
+           q.cells >= solo_cells();
   }
 
   // `n` steps of a solo request.  Fused launches leave the buffers holding

@@ -27,6 +27,7 @@
 
 #include <type_traits>
 
+#include "stream_kernel.hpp"
 #include "tile_kernel.hpp"
 #include "zmarch2_kernel.hpp"
 

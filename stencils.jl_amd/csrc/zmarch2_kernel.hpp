@@ -45,22 +45,28 @@ constexpr int zmarch2_rows() {
 // Lane l receives lane l - 1's value (DX = -1) or lane l + 1's (DX = +1);
 // the lane with no such neighbour receives 0 (it is a rim lane whose result
 // is never used).
-template <int DX>
-__device__ __forceinline__ float lane_shift(float v) {
+template <int DX, typename Val>
+__device__ __forceinline__ Val lane_shift(Val v) {
   // old value undefined and bound_ctrl set: the move can fold into the
   // instruction that consumes it as a DPP operand
   constexpr int ctrl = DX < 0 ? 0x138 /* wave_shr:1 */ : 0x130 /* wave_shl:1 */;
-  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), ctrl, 0xf, 0xf, true));
-}
-template <int DX>
-__device__ __forceinline__ double lane_shift(double v) {
-  constexpr int ctrl = DX < 0 ? 0x138 : 0x130;
-  const uint64_t u = __builtin_bit_cast(uint64_t, v);
-  const uint32_t l2 = static_cast<uint32_t>(
-      __builtin_amdgcn_mov_dpp(static_cast<int>(static_cast<uint32_t>(u)), ctrl, 0xf, 0xf, true));
-  const uint32_t h2 = static_cast<uint32_t>(
-      __builtin_amdgcn_mov_dpp(static_cast<int>(static_cast<uint32_t>(u >> 32)), ctrl, 0xf, 0xf, true));
-  return __builtin_bit_cast(double, (static_cast<uint64_t>(h2) << 32) | l2);
+  if constexpr (sizeof(Val) == 8) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t l2 = static_cast<uint32_t>(
+        __builtin_amdgcn_mov_dpp(static_cast<int>(static_cast<uint32_t>(u)), ctrl, 0xf, 0xf, true));
+    const uint32_t h2 = static_cast<uint32_t>(
+        __builtin_amdgcn_mov_dpp(static_cast<int>(static_cast<uint32_t>(u >> 32)), ctrl, 0xf, 0xf, true));
+    return __builtin_bit_cast(Val, (static_cast<uint64_t>(h2) << 32) | l2);
+  } else if constexpr (sizeof(Val) == 4) {
+    return __builtin_bit_cast(Val, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), ctrl, 0xf, 0xf, true));
+  } else if constexpr (sizeof(Val) == 2) {
+    const int w = __builtin_bit_cast(uint16_t, v);
+    return __builtin_bit_cast(Val, static_cast<uint16_t>(__builtin_amdgcn_mov_dpp(w, ctrl, 0xf, 0xf, true)));
+  } else {
+    static_assert(sizeof(Val) == 1, "1-, 2-, 4- or 8-byte values");
+    const int w = __builtin_bit_cast(uint8_t, v);
+    return __builtin_bit_cast(Val, static_cast<uint8_t>(__builtin_amdgcn_mov_dpp(w, ctrl, 0xf, 0xf, true)));
+  }
 }
 
 template <int DX, typename Val>

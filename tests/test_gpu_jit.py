@@ -81,3 +81,37 @@ def test_specialised_iteration(gpu, oracle):
     S = sh.mapstencil_steps(sh.mean, S, 9, graph=True)
     O = oracle.mapstencil_steps(sh.mean, sh.SwitchingStencilArray(A.copy(order="F"), st, boundary=sh.Wrap()), 9)
     np.testing.assert_array_equal(np.asfortranarray(S.source.cpu().numpy()), np.asarray(O.source))
+
+
+FUSED = [
+    # (stencil, dtype, fold, shape): two steps per launch, runtime-compiled (2^21 cells and more)
+    (sh.VonNeumann(1), np.int32, sh.maximum, (2048, 1536)),
+    (sh.Moore(1), np.int64, sh.sum, (1536, 1536)),
+    (sh.Positional((-1, 0), (0, 0), (1, 1), (0, -1)), np.float32, sh.mean, (2048, 1024)),
+    (sh.VonNeumann(1, 3), np.int32, sh.minimum, (128, 128, 128)),
+    (sh.Moore(1, 3), np.int16, sh.maximum, (128, 136, 128)),
+]
+
+
+@pytest.mark.parametrize("bname", ["remove", "wrap", "reflect"])
+@pytest.mark.parametrize("case", FUSED, ids=lambda c: f"{c[0]!r}-{np.dtype(c[1]).name}-{len(c[3])}d")
+def test_specialised_two_step_iteration(gpu, oracle, case, bname):
+    """Iterations of radius-1 stencils outside the catalogue (integer types,
+    a user offset list) run the two-step kernels compiled at run time: 13
+    steps through sth_iterate (graph replay), both buffers equal to the
+    oracle's step-by-step iteration."""
+    import torch
+    st, dt, f, shape = case
+    rng = np.random.default_rng(len(shape) * 10 + np.dtype(dt).itemsize)
+    if np.dtype(dt).kind == "f":
+        A = np.asfortranarray(rng.random(shape).astype(dt))
+    else:
+        A = np.asfortranarray(rng.integers(-60, 60, shape).astype(dt))
+    bc = {"remove": sh.Remove(np.dtype(dt).type(5)), "wrap": sh.Wrap(), "reflect": sh.Reflect()}[bname]
+    t = sh.colmajor_empty(shape, torch.from_numpy(np.zeros(1, dt)).dtype, torch.device("cuda"))
+    rev = tuple(reversed(range(len(shape))))
+    t.copy_(torch.from_numpy(np.ascontiguousarray(A.transpose(rev))).permute(*rev))
+    S = sh.mapstencil_steps(f, sh.SwitchingStencilArray(t, st, boundary=bc), 13, graph=True)
+    O = oracle.mapstencil_steps(f, sh.SwitchingStencilArray(A.copy(order="F"), st, boundary=bc), 13)
+    np.testing.assert_array_equal(np.asfortranarray(S.source.cpu().numpy()), np.asarray(O.source))
+    np.testing.assert_array_equal(np.asfortranarray(S.dest.cpu().numpy()), np.asarray(O.dest))
