@@ -153,7 +153,8 @@ typedef struct sth_op {
 
 /* Call flags. */
 #define STH_ASYNC 1u     /* return without synchronising the stream */
-#define STH_GRAPH 2u     /* sth_iterate: replay the steps as a captured hipGraph */
+#define STH_GRAPH 2u     /* sth_iterate: replay the steps as a captured hipGraph
+                            (calls of 17 launches or more; shorter ones launch directly) */
 #define STH_GENERIC 4u   /* force the runtime-offset kernel (testing / tuning) */
 #define STH_NO_TUNE 16u  /* use the default kernel variant; never time variants */
 #define STH_NO_FUSE 32u  /* sth_iterate: no temporal blocking (one generation per launch) */

@@ -368,9 +368,10 @@ def test_graph_cache_bound(gpu, oracle):
         G = sh.SwitchingStencilArray(t, sh.Window(1), boundary=sh.Reflect())
         alive.append(G)
         O = sh.SwitchingStencilArray(A.copy(order="F"), sh.Window(1), boundary=sh.Reflect())
-        # 8 steps: two graphs per call (6 replayed steps and 1), 600 in all
-        G = sh.mapstencil_steps(sh.mean, G, 8, graph=True)
-        O = oracle.mapstencil_steps(sh.mean, O, 8)
+        # 20 steps (graphs start at 17 launches): two graphs per call (the
+        # first step with 18 replayed steps, and 1), 600 in all
+        G = sh.mapstencil_steps(sh.mean, G, 20, graph=True)
+        O = oracle.mapstencil_steps(sh.mean, O, 20)
         np.testing.assert_array_equal(G.source.cpu().numpy(), O.source, err_msg=f"call {i}")
 
 
