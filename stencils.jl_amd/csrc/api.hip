@@ -1703,11 +1703,29 @@ int sth_scatter(sth_ctx* ctx, const sth_term* source, const sth_term* aux, const
   if (needs_aux && pa.halo) pa.halo_fn(pa.hp, s);
   STH_HIP(hipGetLastError());
   if (tx * ty > 0) {
+    // large calls: the interior order compiled in (immediate LDS offsets)
+    const JitKernel* jk = nullptr;
+    if (!(flags & STH_GENERIC) && jit_wanted(p.dst.n0 * p.dst.n1)) {
+      std::string why;
+      jk = jit_scatter_kernel(dt, p.kind, p.combine, p.S, L, tbl.data() + static_cast<size_t>(L) * (1 + p.S), &why);
+      if (!jk) jit_note("scatter kernel", why);
+    }
+    hipError_t lerr = hipSuccess;
     rc = with_device_table(ctx, tbl.data(), tbl.size() * sizeof(int32_t), s, [&](const void* dev) {
-      if (dt == STH_F32) launch_scatter<float>(p, static_cast<const int32_t*>(dev), s);
-      else launch_scatter<double>(p, static_cast<const int32_t*>(dev), s);
+      if (jk) {
+        const int32_t* dtbl = static_cast<const int32_t*>(dev);
+        void* args[] = {&p, &dtbl};
+        const unsigned nb = static_cast<unsigned>(p.gen_blocks) + static_cast<unsigned>(p.tiles_x) * p.tiles_y;
+        const size_t lds = scatter_lds_bytes(p.kind, p.R, p.L, dtype_size(dt));
+        lerr = hipModuleLaunchKernel(jk->fn, nb, 1, 1, 256, 1, 1, static_cast<unsigned>(lds), s, args, nullptr);
+      } else if (dt == STH_F32) {
+        launch_scatter<float>(p, static_cast<const int32_t*>(dev), s);
+      } else {
+        launch_scatter<double>(p, static_cast<const int32_t*>(dev), s);
+      }
     });
     if (rc) return rc;
+    STH_HIP(lerr);
   }
   if (!(flags & STH_ASYNC)) STH_HIP(hipStreamSynchronize(s));
   return STH_OK;
@@ -2069,6 +2087,16 @@ int sth_jit_selftest(char* log, size_t cap) {
     jit_fused_source(c[0], c[1], c[2] == 3 ? 6 : 4, c[2] == 3 ? &vn3[0][0] : &vn2[0][0], 1, 1, c[2] == 3 ? 1 : 0, c[2],
                      STH_WRAP, &src, &expr);
     one(src, expr, std::string("two-step ") + std::to_string(c[2]) + "-D " + jit_type_name(c[0]));
+  }
+  {
+    // Moore(1) in a 66-wide box, residues in some order
+    int32_t deltas[3 * 8];
+    for (int i = 0; i < 24; ++i) deltas[i] = ((i % 8) - 4) * 66 + (i % 3) - 1;
+    for (int kind : {STH_SCATTER_CONST, STH_SCATTER_DOWNHILL_FLOW}) {
+      std::string src, expr;
+      jit_scatter_source(STH_F64, kind, STH_COMBINE_ADD, 3, 8, deltas, &src, &expr);
+      one(src, expr, "scatter f64 rule " + std::to_string(kind));
+    }
   }
   if (log && cap) {
     const size_t n = std::min(cap - 1, out.size());

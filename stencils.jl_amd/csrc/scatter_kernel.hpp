@@ -225,6 +225,23 @@ __device__ __forceinline__ T scatter_general(const ScatterParams& p, const int32
   return acc;
 }
 
+// One step of an interior cell's fold with its sender at LDS delta D, known
+// at compile time (runtime-compiled kernels, whose interior order is code).
+template <typename T, int KIND, int COMB, int D>
+__device__ __forceinline__ T scatter_step(T acc, T cf, const T* __restrict__ la, const T* __restrict__ lb, int c, T bd) {
+  T a = T(0), b = T(0);
+  if constexpr (KIND != STH_SCATTER_CONST) {
+    a = la[c + D];
+    if constexpr (KIND >= STH_SCATTER_DOWNHILL) b = lb[c + D];
+  }
+  return scatter_combine<T>(acc, scatter_rule<T, KIND>(a, b, bd, cf), COMB);
+}
+
+// The interior order read from the LDS table (the build-time kernels).
+struct TableOrder {
+  static constexpr bool kStatic = false;
+};
+
 // Copy the box [x0, x0 + W) x [y0, y0 + H) of A (and B) into LDS, zero
 // outside the array: batches of kB elements per thread, every load of a
 // batch issued before the first LDS store, so a workgroup keeps kB loads in
@@ -289,7 +306,7 @@ __device__ __forceinline__ void scatter_band_cell(const ScatterParams& p, const 
   *dp = scatter_general<T, KIND, COMB>(p, fold, acc, static_cast<T>(p.coef), di, dj);
 }
 
-template <typename T, int KIND, int COMB>
+template <typename T, int KIND, int COMB, class ORD = TableOrder>
 __global__ __launch_bounds__(256) void scatter_kernel(ScatterParams p, const int32_t* __restrict__ tbl) {
   extern __shared__ __align__(16) unsigned char smem_dyn[];
   if (static_cast<int>(blockIdx.x) < p.gen_blocks) {
@@ -361,7 +378,10 @@ __global__ __launch_bounds__(256) void scatter_kernel(ScatterParams p, const int
       const int c = (jj + R) * W + lane + R;
       const ScatterPre pj = scatter_preimages(dj, n1, R, p.bc);
       if (row_inner && dj > R && dj < n1 - 1 - R) {
-        acc = scatter_direct<T, KIND, COMB, false>(acc, cf, pk, dl, L, la, lb, c, di, dj, n0, n1);
+        if constexpr (ORD::kStatic)
+          acc = ORD::template run<T, KIND, COMB>(r, acc, cf, la, lb, c);
+        else
+          acc = scatter_direct<T, KIND, COMB, false>(acc, cf, pk, dl, L, la, lb, c, di, dj, n0, n1);
         dst[static_cast<int64_t>(dj) * p.dst.s1 + di] = acc;
       } else if (!row_general && !pj.v0 && !pj.v2) {
         acc = scatter_direct<T, KIND, COMB, true>(acc, cf, pk, dl, L, la, lb, c, di, dj, n0, n1);

@@ -114,6 +114,34 @@ def test_large_radius_against_oracle(gpu, oracle, st, bc):
         np.testing.assert_array_equal(_host(gd), od, err_msg=f"{f} {op} {st} {bc}")
 
 
+@pytest.mark.parametrize("st", [sh.Moore(1), sh.VonNeumann(2), sh.Circle(2)], ids=repr)
+@pytest.mark.parametrize("bc", BOUNDS, ids=["remove", "wrap", "reflect"])
+def test_large_against_oracle(gpu, oracle, st, bc):
+    """Calls of 2^20 cells and more run the kernel with each residue's
+    interior order compiled in at run time: every rule and op at 1024^2 in
+    Float64 and Float32 against the oracle, bit for bit."""
+    rng = np.random.default_rng(len(st.offsets()) * 3 + 1)
+    shape = (1024, 1024)
+    cases = [(Scatter.downhill_flow(0.7), "+", np.float64), (Scatter.downhill(0.3), max, np.float64),
+             (Scatter.center(), min, np.float32), (Scatter.scaled(0.1), "+", np.float32),
+             (Scatter.constant(0.25), "+", np.float64)]
+    for f, op, dt in cases:
+        if dt == np.float32:
+            f = Scatter(f.kind, float(np.float32(f.coef)))
+        A = np.asfortranarray(rng.random(shape).astype(dt))
+        B = np.asfortranarray(rng.random(shape).astype(dt))
+        D = np.asfortranarray(rng.random(shape).astype(dt))
+        gsa = sh.StencilArray(_dev(A), st, boundary=bc)
+        osa = sh.StencilArray(A.copy(order="F"), st, boundary=bc)
+        gaux = [sh.StencilArray(_dev(B), st, boundary=sh.Wrap())] if f.needs_aux() else []
+        oaux = sh.StencilArray(B.copy(order="F"), st, boundary=sh.Wrap()) if f.needs_aux() else None
+        gd = _dev(D)
+        od = D.copy(order="F")
+        sh.scatterstencil_(f, op, gd, gsa, *gaux)
+        oracle.scatterstencil(f, op, od, osa, oaux)
+        np.testing.assert_array_equal(_host(gd), od, err_msg=f"{f} {op} {st} {bc}")
+
+
 def test_switching_forms(gpu, oracle):
     """scatterstencil!(f, op, A::SwitchingStencilArray) clears dest and
     scatters A's source (Conditional); the explicit-source form scatters
