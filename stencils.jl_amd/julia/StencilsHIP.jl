@@ -478,11 +478,12 @@ function _scatter(f::ScatterRule, op, dest::ROCArray{T,2}, source, args, flags::
     (comb === nothing || coef === nothing) && return STH_ENOTSUP
     needs_aux = f isa Union{Downhill,DownhillFlow}
     length(args) == (needs_aux ? 1 : 0) || return STH_ENOTSUP
+    needs_aux && !(args[1] isa AbstractStencilArray{<:Any,<:Any,2,<:ROCArray}) && return STH_ENOTSUP
     keep = Any[]
     ts = _term(sum, source, T, keep)
     ts === nothing && return STH_ENOTSUP
     ta = needs_aux ? _term(sum, args[1], T, keep) : nothing
-    needs_aux && (ta === nothing || !(args[1] isa AbstractStencilArray)) && return STH_ENOTSUP
+    needs_aux && ta === nothing && return STH_ENOTSUP
     rule = SthScatterRule(_rule_kind(f), comb, coef)
     da = _array(dest, size(dest), 0)
     return GC.@preserve keep ccall((:sth_scatter, libsth), Cint,
@@ -633,6 +634,7 @@ function mapstencil_steps_batch!(items::AbstractVector; graph::Bool=true)
         A
     end
     for (f, A, k) in items
+        (A isa SwitchingStencilArray && source(A) isa ROCArray && k >= 0) || return fallback()
         T = eltype(A)
         op = haskey(DTYPES, T) ? _op(f, T) : nothing
         pv = _padval(boundary(A), T)
