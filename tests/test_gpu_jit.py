@@ -93,9 +93,10 @@ FUSED = [
 ]
 
 
+@pytest.mark.parametrize("pad", ["cond", "halo"])
 @pytest.mark.parametrize("bname", ["remove", "wrap", "reflect"])
 @pytest.mark.parametrize("case", FUSED, ids=lambda c: f"{c[0]!r}-{np.dtype(c[1]).name}-{len(c[3])}d")
-def test_specialised_two_step_iteration(gpu, oracle, case, bname):
+def test_specialised_two_step_iteration(gpu, oracle, case, bname, pad):
     """Iterations of radius-1 stencils outside the catalogue (integer types,
     a user offset list) run the two-step kernels compiled at run time: 13
     steps through sth_iterate (graph replay), both buffers equal to the
@@ -111,7 +112,8 @@ def test_specialised_two_step_iteration(gpu, oracle, case, bname):
     t = sh.colmajor_empty(shape, torch.from_numpy(np.zeros(1, dt)).dtype, torch.device("cuda"))
     rev = tuple(reversed(range(len(shape))))
     t.copy_(torch.from_numpy(np.ascontiguousarray(A.transpose(rev))).permute(*rev))
-    S = sh.mapstencil_steps(f, sh.SwitchingStencilArray(t, st, boundary=bc), 13, graph=True)
-    O = oracle.mapstencil_steps(f, sh.SwitchingStencilArray(A.copy(order="F"), st, boundary=bc), 13)
+    padding = sh.Conditional() if pad == "cond" else sh.Halo()
+    S = sh.mapstencil_steps(f, sh.SwitchingStencilArray(t, st, boundary=bc, padding=padding), 13, graph=True)
+    O = oracle.mapstencil_steps(f, sh.SwitchingStencilArray(A.copy(order="F"), st, boundary=bc, padding=padding), 13)
     np.testing.assert_array_equal(np.asfortranarray(S.source.cpu().numpy()), np.asarray(O.source))
     np.testing.assert_array_equal(np.asfortranarray(S.dest.cpu().numpy()), np.asarray(O.dest))

@@ -190,6 +190,34 @@ def test_served_2d_requests_run_solo_two_step_kernels(gpu, oracle):
         np.testing.assert_array_equal(c.result.cpu().numpy(), O.source, err_msg=f"{f}")
 
 
+def test_served_integer_requests_run_runtime_compiled_two_step_kernels(gpu, oracle):
+    """Int32 requests of 2^21 cells (maximum over VonNeumann(1), Wrap; a
+    user offset list under Reflect) have no build-time two-step kernel: they
+    run solo on the two-step kernels compiled at run time, and their results
+    equal the step-by-step oracle."""
+    import torch
+    dev = torch.device("cuda")
+    rng = np.random.default_rng(6)
+    A = rng.integers(-500, 500, (2048, 1024)).astype(np.int32)
+    B = rng.integers(-500, 500, (1536, 1536)).astype(np.int32)
+    cases = [(sh.maximum, A, sh.VonNeumann(1), sh.Wrap(), 22),
+             (sh.minimum, B, sh.Positional((-1, 1), (0, 0), (1, 0), (0, -1)), sh.Reflect(), 19)]
+    reqs, hosts = [], []
+    for f, X, st, bc, steps in cases:
+        t = sh.colmajor_empty(X.shape, torch.int32, dev)
+        t.copy_(torch.from_numpy(X))
+        reqs.append((f, sh.SwitchingStencilArray(t, st, boundary=bc), steps, 0.0))
+        hosts.append(sh.SwitchingStencilArray(np.asfortranarray(X), st, boundary=bc))
+    done, tags, stats = _serve(reqs, hbm_budget=1 << 30, round_steps=4)
+    assert len(done) == 2
+    assert stats["fused_launches"] >= 2, stats
+    for c in done:
+        f, S, steps = tags[c.tag]
+        k = next(i for i, r in enumerate(reqs) if r[1] is S)
+        O = oracle.mapstencil_steps(f, hosts[k], steps)
+        np.testing.assert_array_equal(c.result.cpu().numpy(), O.source, err_msg=f"{f}")
+
+
 def test_invalid_request_rejected_at_submit(gpu):
     """A request the library cannot run (reach >= axis, the reference's
     ArgumentError) is rejected when submitted, through the C ABI past the
