@@ -229,11 +229,14 @@ def bench_multi(sh, torch, n: int = 8192):
     g = sh.Terms(sh.mean, sh.maximum, sh.sum)
     out = sh.colmajor_empty((n, n), torch.float32, dev)
     t_three = timed(lambda: sh.mapstencil_(g, out, *args, flags=sh._abi.ASYNC))
+    t_passes = timed(lambda: sh.mapstencil_(g, out, *args, flags=sh._abi.ASYNC | sh._abi.NO_FUSE))
     cells = n * n
     return {"workload": f"coupled fields {n}x{n} Float32: heat with a source (Laplacian(0.2) + 0.05 * source), "
                         "and three StencilArrays (mean + maximum + sum)",
             "heat_source_us": t_heat * 1e6, "heat_source_gbs": 12 * cells / t_heat / 1e9,
-            "three_arrays_us": t_three * 1e6, "three_arrays_gbs": 16 * cells / t_three / 1e9}
+            "three_arrays_us": t_three * 1e6, "three_arrays_gbs": 16 * cells / t_three / 1e9,
+            "three_arrays_one_pass_per_argument_us": t_passes * 1e6,
+            "three_arrays_fused_speedup": t_passes / t_three}
 
 
 def bench_scatter(sh, torch, n: int = 8192, oracle_ok: bool = False):

@@ -1501,15 +1501,87 @@ int sth_map_multi(sth_ctx* ctx, int32_t nterms, const sth_term* terms, const sth
     for (int32_t t : folds)
       ok = ok && std::max(tbl[t].rx, std::max(tbl[t].ry, tbl[t].rz)) <= kJitMaxReach && tbl[t].opk != STH_OP_LIFE;
     if (!ok) jit_note("multi-array call", "no stencil argument, a reach above 3 or the Life rule");
+    const int es = dtype_size(dt);
+    const int v = es == 1 ? 8 : 16 / es;
+    // Two stencil arguments or more, none weighted: one kernel for the
+    // call, every argument's fold and the centre terms combined in registers
+    // (STH_NO_FUSE keeps one pass per stencil argument).
+    bool fused_ok = ok && folds.size() >= 2 && !(flags & STH_NO_FUSE);
+    for (int32_t t : folds) fused_ok = fused_ok && tbl[t].opk != STH_OP_KERNELPRODUCT;
+    if (fused_ok) {
+      int ky = 4;
+      std::vector<std::vector<int8_t>> o8s;
+      std::vector<JitFold> jf;
+      std::string items;
+      for (int32_t u = 0; u < nterms; ++u) items.push_back(tbl[u].direct ? (tbl[u].scaled ? 'C' : 'c')
+                                                                          : (tbl[u].scaled ? 'F' : 'f'));
+      for (int32_t t : folds) {
+        const MultiTerm& q = tbl[t];
+        if (q.rz == 0 && std::max(q.rx, q.ry) > 2) ky = 2;
+        std::vector<int8_t> o8(3 * static_cast<size_t>(q.L));
+        for (int k = 0; k < q.L; ++k) {
+          o8[3 * k] = static_cast<int8_t>(q.offs[k] & 0xff);
+          o8[3 * k + 1] = static_cast<int8_t>((q.offs[k] >> 8) & 0xff);
+          o8[3 * k + 2] = static_cast<int8_t>((q.offs[k] >> 16) & 0xff);
+        }
+        o8s.push_back(std::move(o8));
+      }
+      for (size_t f = 0; f < folds.size(); ++f) {
+        const MultiTerm& q = tbl[folds[f]];
+        jf.push_back(JitFold{q.opk, q.L, q.rx, q.ry, q.rz, o8s[f].data()});
+      }
+      const int lsh = lpr_shift_for(n0, v);
+      const int64_t lpr = int64_t(1) << lsh, rpw = 64 / lpr;
+      const int64_t jtx = (n0 + lpr * v - 1) / (lpr * v), jty = (n1 + 4 * ky * rpw - 1) / (4 * ky * rpw);
+      std::string why;
+      const JitKernel* k = nullptr;
+      if (jtx * jty * n2 <= (int64_t(1) << 31) - 1 && n2 <= (int64_t(1) << 30))
+        k = jit_multi_fused(dt, v, ky, jf, items, combine == STH_COMBINE_MUL, &why);
+      else
+        why = "array too large for one launch";
+      if (k) {
+        StreamMultiParams sp;
+        std::memset(static_cast<void*>(&sp), 0, sizeof sp);
+        int nc = 0;
+        for (int32_t u = 0; u < nterms; ++u) {
+          sp.coef[u] = tbl[u].coef;
+          if (tbl[u].direct) {
+            sp.cbase[nc] = tbl[u].src.base;
+            sp.cs1[nc] = tbl[u].src.s1;
+            sp.cs2[nc] = tbl[u].src.s2;
+            ++nc;
+          }
+        }
+        for (size_t f = 0; f < folds.size(); ++f) {
+          const MapParams& pp = plans[folds[f]].p;
+          StreamTerm& st = sp.term[f];
+          st.src = pp.src;
+          st.dst = pp.dst;
+          st.bc = pp.bc;
+          st.read_halo = pp.read_halo;
+          st.lpr_shift = lsh;
+          st.padval = pp.padval;
+          st.op = pp.op;
+        }
+        sp.tiles_x = static_cast<int32_t>(jtx);
+        sp.tiles_y = static_cast<int32_t>(jty);
+        sp.tiles_z = static_cast<int32_t>(n2);
+        sp.xcd_remap = 1;
+        void* args[] = {&sp};
+        STH_HIP(hipModuleLaunchKernel(k->fn, static_cast<unsigned>(jtx * jty * n2), 1, 1, 256, 1, 1, 0, s, args,
+                                      nullptr));
+        done = true;
+      } else {
+        jit_note("fused multi-array kernel", why);
+      }
+    }
     struct Pass {
       const JitKernel* k;
       MapParams p;
       MultiEpiParams e;
     };
     std::vector<Pass> passes;
-    const int es = dtype_size(dt);
-    const int v = es == 1 ? 8 : 16 / es;
-    for (size_t f = 0; ok && f < folds.size(); ++f) {
+    for (size_t f = 0; ok && !done && f < folds.size(); ++f) {
       const int32_t t = folds[f];
       const int32_t lo = f == 0 ? 0 : t, hi = f + 1 < folds.size() ? folds[f + 1] : nterms;
       const MultiTerm& q = tbl[t];
@@ -1562,7 +1634,7 @@ int sth_map_multi(sth_ctx* ctx, int32_t nterms, const sth_term* terms, const sth
       ps.p.tiles_z = static_cast<int32_t>(n2);
       passes.push_back(ps);
     }
-    if (ok) {
+    if (ok && !done) {
       for (const Pass& ps : passes) STH_HIP(jit_launch_multi(ps.k->fn, ps.p, ps.e, s));
       done = true;
     }
@@ -2080,6 +2152,26 @@ int sth_jit_selftest(char* log, size_t cap) {
   }
   static const int8_t vn2[4][3] = {{0, -1, 0}, {-1, 0, 0}, {1, 0, 0}, {0, 1, 0}};
   static const int8_t vn3[6][3] = {{0, 0, -1}, {0, -1, 0}, {-1, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  {
+    // multi-array kernels with every argument in one pass
+    const std::vector<JitFold> two = {{STH_OP_LAPLACIAN, 4, 1, 1, 0, &vn2[0][0]}, {STH_OP_MAX, 6, 2, 2, 0, &shape[0][0]}};
+    const std::vector<JitFold> three = {{STH_OP_SUM, 6, 1, 1, 1, &vn3[0][0]}, {STH_OP_MIN, 4, 1, 1, 0, &vn2[0][0]},
+                                        {STH_OP_MEAN, 6, 1, 1, 1, &vn3[0][0]}};
+    struct F {
+      int dt;
+      const std::vector<JitFold>* folds;
+      const char* items;
+      bool mul;
+    };
+    const F fs[] = {{STH_F32, &two, "CfFc", false}, {STH_F64, &three, "fcfF", true}, {STH_U8, &two, "fcf", true},
+                    {STH_I64, &three, "cfff", false}};
+    for (const F& m : fs) {
+      std::string src, expr;
+      const int v = m.dt == STH_U8 ? 8 : 16 / dtype_size(m.dt);
+      jit_multi_fused_source(m.dt, v, 4, *m.folds, m.items, m.mul, &src, &expr);
+      one(src, expr, std::string("fused multi ") + jit_type_name(m.dt) + " " + m.items);
+    }
+  }
   const int fdt[][3] = {{STH_I32, STH_OP_MAX, 2}, {STH_I64, STH_OP_SUM, 2}, {STH_F32, STH_OP_SUM, 2},
                         {STH_I16, STH_OP_MIN, 3}, {STH_I64, STH_OP_SUM, 3}, {STH_F64, STH_OP_MAX, 3}};
   for (const auto& c : fdt) {

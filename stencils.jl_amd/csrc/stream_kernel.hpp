@@ -93,12 +93,15 @@ __device__ __forceinline__ int64_t resolve_row(int64_t y, int64_t n, int64_t h, 
   return map_index(yc, n, bc);
 }
 
-// What happens to a lane's V results of one row before they are stored:
-// nothing (a plain gatherstencil!), or the combination with further terms
-// of a multi-array gather (runtime-compiled kernels define their own).
+// What happens to a lane's V results of row j (row y of the array) before
+// they are stored: nothing (a plain gatherstencil!), or the combination with
+// further terms of a multi-array gather (runtime-compiled kernels define
+// their own).  An epilogue with kStore false keeps the results itself and
+// nothing is stored.
 struct NoEpilogue {
+  static constexpr bool kStore = true;
   template <typename R, int V>
-  __device__ __forceinline__ void apply(RawVec<R, V>&, int64_t, int64_t, int64_t, int64_t) const {}
+  __device__ __forceinline__ void apply(RawVec<R, V>&, int64_t, int64_t, int64_t, int64_t, int) const {}
 };
 
 template <typename S, int OPK, class Sh, int V, int KY, class P, class Epi = NoEpilogue>
@@ -249,17 +252,19 @@ __device__ __forceinline__ void stream_body(const P& p, int bx, int by, int bz, 
     }
     const int64_t y = y0 + j;
     if (y < n1) {
-      epi.apply(out, x0, y, z0, n0);
-      ORaw* drow = dbase + y * p.dst.s1;
-      if (x0 + V <= n0) {
-        if (dst_aligned)
-          store_avec<ORaw, V>(drow + x0, out);
-        else
-          store_vec<ORaw, V>(drow + x0, out);
-      } else {
+      epi.apply(out, x0, y, z0, n0, j);
+      if constexpr (Epi::kStore) {
+        ORaw* drow = dbase + y * p.dst.s1;
+        if (x0 + V <= n0) {
+          if (dst_aligned)
+            store_avec<ORaw, V>(drow + x0, out);
+          else
+            store_vec<ORaw, V>(drow + x0, out);
+        } else {
 #pragma unroll
-        for (int i = 0; i < V; ++i)
-          if (x0 + i < n0) drow[x0 + i] = out.e[i];
+          for (int i = 0; i < V; ++i)
+            if (x0 + i < n0) drow[x0 + i] = out.e[i];
+        }
       }
     }
   }
@@ -290,6 +295,32 @@ __global__ __launch_bounds__(256) void stream_kernel_epi(MapParams p, typename E
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
   stream_body<S, OPK, Sh, V, KY>(p, bx, by, bz, static_cast<int>(threadIdx.x) & 63, wave, Epi{e});
 }
+
+// One stencil argument of a multi-array call whose arguments are fused into
+// one streaming kernel: what stream_body reads of MapParams (the offsets are
+// compiled in; weighted folds are not fused, so no weight table).
+struct StreamTerm {
+  DevArr src;
+  DevArr dst;
+  int32_t bc;
+  int32_t read_halo;
+  int32_t lpr_shift;
+  int32_t pad_;
+  uint64_t padval;
+  OpParams op;
+};
+
+// Parameters of such a kernel (runtime-compiled: the fold and centre terms
+// are code): the stencil arguments in call order, the centre arguments in
+// call order (interior element (0,0,0) and strides), each argument's
+// coefficient by its position in the call, and the tile grid.
+struct StreamMultiParams {
+  StreamTerm term[STH_MAX_TERMS];
+  const void* cbase[STH_MAX_TERMS];
+  int64_t cs1[STH_MAX_TERMS], cs2[STH_MAX_TERMS];
+  double coef[STH_MAX_TERMS];
+  int32_t tiles_x, tiles_y, tiles_z, xcd_remap;
+};
 
 // Natural vector width: 16 bytes per lane.
 template <typename S>

@@ -158,16 +158,24 @@ LARGE = {2: [(1024, 1024), (1100, 960)], 3: [(128, 128, 64)]}
 @pytest.mark.parametrize("seed", range(4))
 @pytest.mark.parametrize("nd", [2, 3])
 @pytest.mark.parametrize("dt", [np.float32, np.float64, np.uint8, np.int32], ids=lambda d: np.dtype(d).name)
-def test_random_multi_large_against_oracle(gpu, oracle, int_wrap, dt, nd, seed):
-    """Calls of 2^20 cells and more run one runtime-compiled streaming pass
-    per stencil argument (the running value read back from dest between
-    passes, centre terms folded in on the way): random Terms as above, the
-    result and every Halo parent equal to the oracle's."""
+@pytest.mark.parametrize("mode", ["fused", "passes"])
+def test_random_multi_large_against_oracle(gpu, oracle, int_wrap, dt, nd, seed, mode):
+    """Calls of 2^20 cells and more run runtime-compiled streaming kernels:
+    one kernel for the whole call when two or more stencil arguments are
+    unweighted ("fused": every fold and centre term combined in registers),
+    otherwise, or with STH_NO_FUSE ("passes"), one pass per stencil argument
+    (the running value read back from dest between passes, centre terms
+    folded in on the way).  Random Terms as above: the result and every Halo
+    parent equal to the oracle's."""
     rng = np.random.default_rng(5000 + 1000 * nd + 10 * seed + np.dtype(dt).itemsize)
     f, arrays, specs = _random_case(rng, dt, nd, LARGE)
     gargs = _args(arrays, specs, device=True)
     oargs = _args(arrays, specs, device=False)
-    g = sh.mapstencil(f, *gargs)
+    if mode == "fused":
+        g = sh.mapstencil(f, *gargs)
+    else:
+        g = _dev(np.zeros(arrays[0].shape, dt))  # `arrays` hold the logical (unpadded) fields
+        sh.mapstencil_(f, g, *gargs, flags=sh._abi.NO_FUSE)
     o = oracle.mapstencil_multi(f, tuple(oargs))
     np.testing.assert_array_equal(_host(g), o, err_msg=f"{f} {[s[:2] if s else None for s in specs]}")
     for ga, oa in zip(gargs, oargs):
