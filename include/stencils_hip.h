@@ -157,7 +157,8 @@ typedef struct sth_op {
                             (calls of 17 launches or more; shorter ones launch directly) */
 #define STH_GENERIC 4u   /* force the runtime-offset kernel (testing / tuning) */
 #define STH_NO_TUNE 16u  /* use the default kernel variant; never time variants */
-#define STH_NO_FUSE 32u  /* sth_iterate: no temporal blocking (one generation per launch) */
+#define STH_NO_FUSE 32u  /* sth_iterate: no temporal blocking (one generation per launch);
+                            sth_map_multi: one pass per stencil argument */
 
 typedef struct sth_ctx sth_ctx;
 
