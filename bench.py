@@ -821,8 +821,9 @@ def main():
         "dtype": "u8",
         "data": "synthetic",
         "config": {"workload": f"game-of-life {n}x{n} UInt8 Bernoulli(0.3) seed 2, Moore(1), Wrap, "
-                               "Halo, B3/S23; SwitchingStencilArray resident in HBM; hipGraph-replayed "
-                               "steps", "cells": cells, "step": "one generation"},
+                               "Halo, B3/S23; SwitchingStencilArray resident in HBM; all steps in one "
+                               "sth_iterate call with STH_GRAPH (replayed as hipGraphs from 17 launches, "
+                               "launched directly below)", "cells": cells, "step": "one generation"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": profiled_traffic("lifeb_kernel_gol_8192.md"),
