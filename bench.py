@@ -176,6 +176,19 @@ def bench_mean_blur(sh, torch, oracle_ok: bool):
            "kernel_us": per_call * 1e6, "cells_per_s": cells / per_call,
            "gbs": 16 * cells / per_call / 1e9, "call_us_python": e2e * 1e6,
            "reference_published_ms": 2.755}
+    # the same call at 16384^2 (PCG64 seed 11), where bandwidth, not launch
+    # latency, bounds it
+    big = 16384
+    tb = sh.colmajor_empty((big, big), torch.float64, dev)
+    tb.copy_(torch.from_numpy(np.random.Generator(np.random.PCG64(11)).random((big, big))).to(dev).T)
+    sb = sh.StencilArray(tb, sh.Window(1), boundary=sh.Remove(0.0))
+    db = sh.colmajor_empty((big, big), torch.float64, dev)
+    big_call = graph_time(torch, lambda: sh.mapstencil_(sh.mean, db, sb, flags=sh._abi.ASYNC), 5)
+    out["large"] = {"workload": f"mean blur {big}x{big} Float64, Window(1), Remove(0.0), Conditional",
+                    "kernel_us": big_call * 1e6, "cells_per_s": big * big / big_call,
+                    "gbs": 16 * big * big / big_call / 1e9,
+                    "frac_of_peak": 16 * big * big / big_call / 1e9 / HBM_PEAK_GBS}
+    del tb, sb, db
     if oracle_ok:
         import oracle
         B = np.asfortranarray(A)

@@ -4,7 +4,8 @@ Each test drives the HIP path through the same entry points, layouts, step
 schedules and element types as the corresponding benchmark workload and
 compares every result with the oracle on the same inputs:
 
-  - mean blur 1000 x 1000 Float64, Window(1), Remove(0.0), Conditional;
+  - mean blur 1000 x 1000 Float64, Window(1), Remove(0.0), Conditional,
+    and the same call at 16384 x 16384;
   - Game of Life 8192 x 8192 (full size) and 1024 x 1024 x 1000 generations,
     Wrap, Halo, through the hipGraph schedules the iteration engine uses for
     67 steps and more (replayed 16-unit two-generation graphs, and 32-step
@@ -61,6 +62,20 @@ def test_mean_blur_1000_f64(gpu, oracle):
     o = oracle.mapstencil(sh.mean, sh.StencilArray(np.asfortranarray(A), sh.Window(1), boundary=sh.Remove(0.0)),
                           nthreads=_threads(oracle))
     np.testing.assert_allclose(g, o, rtol=1e-14, atol=0)
+    np.testing.assert_array_equal(g, o)
+
+
+def test_mean_blur_16384_f64(gpu, oracle):
+    """The same call on 16384 x 16384 Float64 (PCG64 seed 11; 2 GiB, past
+    32-bit byte offsets), the size the single-array bandwidth figure is
+    quoted at: bit-exact against the oracle."""
+    import torch
+    A = np.random.Generator(np.random.PCG64(11)).random((16384, 16384))
+    sa = sh.StencilArray(_to_device(sh, torch, A), sh.Window(1), boundary=sh.Remove(0.0))
+    g = sh.mapstencil(sh.mean, sa).cpu().numpy()
+    del sa
+    o = oracle.mapstencil(sh.mean, sh.StencilArray(np.asfortranarray(A), sh.Window(1), boundary=sh.Remove(0.0)),
+                          nthreads=_threads(oracle))
     np.testing.assert_array_equal(g, o)
 
 
