@@ -233,9 +233,10 @@ def test_parity_wide_aligned_rows(gpu, oracle, dt):
         _lib.set_variant("")
 
 
+@pytest.mark.parametrize("variant", ["", "stream4x4"])
 @pytest.mark.parametrize("shape", [(96, 77), (1100, 23)])
 @pytest.mark.parametrize("tname", ["bfloat16", "float16"])
-def test_parity_storage_types(gpu, oracle, tname, shape):
+def test_parity_storage_types(gpu, oracle, tname, shape, variant):
     """BF16/F16 storage, f32 accumulation, one rounding at the store: bit-exact
     against the oracle's restatement of the same arithmetic (the wide shape's
     interior waves take the aligned three-vector path)."""
@@ -246,6 +247,15 @@ def test_parity_storage_types(gpu, oracle, tname, shape):
     raw = At.view(torch.int16).numpy().view(np.uint16)
     code = sh._abi.BF16 if tname == "bfloat16" else sh._abi.F16
     dev = torch.device("cuda")
+    from stencils_hip import _lib
+    _lib.set_variant(variant)  # "stream4x4": four rows per lane where a shape has that variant
+    try:
+        _storage_cases(torch, tdt, A32, At, raw, code, dev, oracle)
+    finally:
+        _lib.set_variant("")
+
+
+def _storage_cases(torch, tdt, A32, At, raw, code, dev, oracle):
     for st in (sh.Window(3), sh.VonNeumann(1), sh.Circle(3), sh.Cross(1)):
         for bname, bc in (("wrap", sh.Wrap()), ("reflect", sh.Reflect()), ("remove", sh.Remove(0.0))):
             for f in (sh.sum, sh.mean, sh.maximum, sh.Laplacian(0.2)):
