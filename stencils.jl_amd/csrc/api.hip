@@ -531,7 +531,9 @@ int make_plan(const sth_array* src, const sth_array* dst, const sth_stencil* st,
         int v = 16 / es;
         if (es == 2 && (rmax >= 2 || reach[2] > 0)) v = 4;
         if (es == 1) v = 8;  // 16 one-byte cells per lane take 140+ VGPRs
-        const int ky = (reach[2] > 0 || rmax <= 2) ? 4 : 2;
+        // four rows per lane (two at reach 3 for 1- and 8-byte cells, whose
+        // value box would take too many registers)
+        const int ky = (reach[2] > 0 || rmax <= 2 || es == 2 || es == 4) ? 4 : 2;
         std::vector<int8_t> o8(3 * L);
         for (int i = 0; i < 3 * L; ++i) o8[i] = static_cast<int8_t>(offs3[i]);
         std::string why;
