@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdlib>
 #include <cstdio>
@@ -893,21 +894,31 @@ int tune_pick(sth_ctx* ctx, const Plan& pl, const std::vector<Variant>& c, const
     return fail(STH_EDEVICE, "hipEventCreate failed");
   }
   int rc = STH_OK;
-  // two interleaved passes (clocks settle during the first), best of each
+  // two interleaved passes (clocks settle during the first), the best time
+  // per launch of each variant; the second pass times enough launches to
+  // span about a quarter of a millisecond, so that kernels of a few
+  // microseconds are not compared on noise
   std::vector<float> best(c.size(), 0.0f);
+  int reps = 3;
   for (int pass = 0; pass < 2 && rc == STH_OK; ++pass) {
     for (size_t i = 0; i < c.size() && rc == STH_OK; ++i) {
       Plan t = pl;
       apply_variant(t, c[i]);
       if (pass == 0) rc = run_plan(t, s);  // warm-up (first launch loads the code object)
       if (rc == STH_OK && hipEventRecord(e0, s) != hipSuccess) rc = fail(STH_EDEVICE, "hipEventRecord failed");
-      for (int r = 0; r < 3 && rc == STH_OK; ++r) rc = run_plan(t, s);
+      for (int r = 0; r < reps && rc == STH_OK; ++r) rc = run_plan(t, s);
       if (rc == STH_OK && hipEventRecord(e1, s) != hipSuccess) rc = fail(STH_EDEVICE, "hipEventRecord failed");
       if (rc == STH_OK && hipEventSynchronize(e1) != hipSuccess) rc = fail(STH_EDEVICE, "hipEventSynchronize failed");
       float ms = 0.0f;
       if (rc == STH_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
         rc = fail(STH_EDEVICE, "hipEventElapsedTime failed");
-      if (rc == STH_OK && (pass == 0 || ms < best[i])) best[i] = ms;
+      const float per = ms / static_cast<float>(reps);
+      if (rc == STH_OK && (pass == 0 || per < best[i])) best[i] = per;
+    }
+    if (pass == 0) {
+      const float fastest = *std::min_element(best.begin(), best.end());
+      const float want = fastest > 0.0f ? 0.25f / fastest : 64.0f;
+      reps = static_cast<int>(std::min(64.0f, std::max(3.0f, std::ceil(want))));
     }
   }
   int bi = 0;
